@@ -1,0 +1,247 @@
+#!/usr/bin/env python3
+"""Benchmark of the voxel pipeline on one MI355X.
+
+Workload (one step): a full remesh -- cull, level-0 splat list and LOD levels
+1..4 -- of every chunk of the 2048x256x2048 uint8 world (4096 chunks of 64^3,
+seeded value-noise terrain with 5% caves), as one batched persistent kernel
+captured in a hipGraph.  Inputs are resident in HBM before timing starts.
+
+Also measured (not the headline value):
+  * the plain per-chunk path: one launch per chunk from the C host loop;
+  * splat resolve of the level-0 list at 1920x1080 over 16 seeded poses;
+  * the roofline of the remesh kernel from HIP events around the timed steps;
+  * the CPU oracle (4 OpenMP threads, as chunkset_manage's team) on the same
+    4096 chunks.
+
+Prints one JSON line.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import pathlib
+import re
+import sys
+import time
+
+ROOT = pathlib.Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBS = 8000.0
+VOXELS_PER_CHUNK = 64 ** 3
+
+
+def log(msg: str) -> None:
+    sys.stderr.write(f"[bench] {msg}\n")
+    sys.stderr.flush()
+
+
+def algorithmic_bytes(uniform, counts) -> int:
+    """Bytes the remesh must move per launch: chunk + 3 halo planes (mixed) or
+    the 3 halo planes alone (uniform), 8 B per record written, 4 B per record
+    of shadow-map reads, 5 u32 counts per chunk."""
+    import numpy as np
+
+    mixed = int((uniform < 0).sum())
+    uni = len(uniform) - mixed
+    records = int(counts.astype(np.int64).sum()) // 4
+    return mixed * (262144 + 3 * 4096) + uni * 3 * 4096 + 12 * records + 20 * len(uniform)
+
+
+def measured_traffic():
+    """HBM bytes per launch measured with rocprofv3 --pmc (committed summary)."""
+    p = ROOT / "profiles" / "cull_mesh_2048x256x2048_traffic.md"
+    if not p.exists():
+        return None
+    m = re.search(r"traffic_bytes_per_launch:\s*([0-9]+)", p.read_text())
+    return int(m.group(1)) if m else None
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-splat", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--cpu-threads", type=int, default=4)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported "
+              "(the pipeline runs on a single MI355X; see DESIGN.md for how it would shard)",
+              file=sys.stderr)
+        return 2
+
+    import numpy as np
+    import torch
+
+    from voxplat_amd import GPU, RemeshGraph, abi
+    from voxplat_amd import workload as W
+
+    dev = "cuda:0"
+    gpu = GPU(0)
+    stream = torch.cuda.current_stream()
+    cfg = W.CFG1
+    w = cfg.desc()
+    nch = w.chunks
+    log(f"generating {cfg.name}: {nch} chunks")
+    vox = torch.empty(nch * VOXELS_PER_CHUNK, dtype=torch.uint8, device=dev)
+    uni = torch.empty(nch, dtype=torch.int16, device=dev)
+    gpu.generate(w, cfg.gen(), vox, uni)
+    sx, sz = w.shadow_dims()
+    smap = torch.empty(abi.shadow_padded_len(sx, sz), dtype=torch.int16, device=dev)
+    gpu.shadow_build(w, vox, smap)
+    ids = torch.arange(nch, dtype=torch.int32, device=dev)
+    _, offs, counts = gpu.cull_mesh(w, vox, ids, shadow_map=smap, uniform=uni, count_only=True)
+    torch.cuda.synchronize()
+    total_items = int(offs[-1].item())
+    cap = total_items // 4
+    splats = torch.empty((cap, 4), dtype=torch.int16, device=dev)
+    uni_h = uni.cpu().numpy()
+    counts_h = counts.cpu().numpy().view(np.uint32)
+    occupancy = {"uniform_air": int((uni_h == 0).sum()), "uniform_solid": int((uni_h > 0).sum()),
+                 "mixed": int((uni_h < 0).sum())}
+    log(f"occupancy {occupancy}, {cap} splats (L0..L4)")
+
+    tune = None
+    if not args.no_autotune:
+        b, wg, ms = gpu.autotune_remesh(w, vox, ids, smap, splats, cap, offs, counts,
+                                        uniform=uni, reps=3)
+        tune = {"block": b, "wg_per_cu": wg, "ms": round(ms, 4)}
+        log(f"autotune: block {b} x {wg}/CU -> {ms:.3f} ms")
+    graph = RemeshGraph(gpu, w, vox, ids, smap, offs, counts, splats, cap, uniform=uni)
+
+    # ---- batched, graph replay: the headline ----
+    for _ in range(args.warmup):
+        graph.launch()
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    e0.record(stream)
+    for _ in range(args.steps):
+        graph.launch()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ev_ms = e0.elapsed_time(e1) / args.steps
+    ms_per_step = max(wall * 1e3 / args.steps, ev_ms)
+    gvox = nch * VOXELS_PER_CHUNK / (ms_per_step * 1e-3) / 1e9
+    assert gpu.status() == 0
+    log(f"batched: {ms_per_step:.3f} ms/step ({ev_ms:.3f} ms by events) -> {gvox:.1f} Gvoxel/s")
+
+    # ---- plain per-chunk path (C host loop, one launch per chunk) ----
+    offs_h = offs.cpu().numpy().view(np.uint64)
+    rec_off = (offs_h[:-1] // 4).astype(np.uint64)
+    rec_cap = ((offs_h[1:] - offs_h[:-1]) // 4).astype(np.uint64)
+    splats_pc = torch.empty_like(splats)
+    offs2 = torch.empty(2 * nch, dtype=torch.int64, device=dev)
+    counts_pc = torch.empty_like(counts)
+    gpu.remesh_per_chunk(w, vox, ids, smap, splats_pc, rec_off, rec_cap, offs2, counts_pc,
+                         uniform=uni)
+    torch.cuda.synchronize()
+    reps = 2
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        gpu.remesh_per_chunk(w, vox, ids, smap, splats_pc, rec_off, rec_cap, offs2, counts_pc,
+                             uniform=uni)
+    torch.cuda.synchronize()
+    pc_ms = (time.perf_counter() - t0) * 1e3 / reps
+    pc_gvox = nch * VOXELS_PER_CHUNK / (pc_ms * 1e-3) / 1e9
+    same = bool(torch.equal(splats_pc, splats)) and bool(torch.equal(counts_pc, counts))
+    log(f"per-chunk: {pc_ms:.2f} ms/pass -> {pc_gvox:.2f} Gvoxel/s; identical output: {same}")
+
+    # ---- roofline of the remesh kernel ----
+    algo = algorithmic_bytes(uni_h, counts_h)
+    achieved = algo / (ev_ms * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": measured_traffic(), "algorithmic_bytes": algo,
+                "kernel_ms": round(ev_ms, 4)}
+
+    # ---- splat resolve at 1080p (level-0 list, chunk order) ----
+    splat_res = None
+    if not args.no_splat:
+        lvl = np.repeat(np.tile(np.arange(5, dtype=np.uint8), nch), (counts_h.reshape(-1) // 4))
+        l0_idx = torch.from_numpy(np.flatnonzero(lvl == 0)).to(dev)
+        l0 = splats.index_select(0, l0_idx).contiguous()
+        n0 = int(l0.shape[0])
+        vh_cols = vox.view(32, 4, 32, 64, 64, 64)          # [cz, cy, cx, z, y, x]
+
+        def top(x, z):
+            col = vh_cols[z >> 6, :, x >> 6, z & 63, :, x & 63].reshape(-1)
+            nz = torch.nonzero(col).reshape(-1)
+            return int(nz.max().item()) + 1 if nz.numel() else 0
+        poses = W.splat_poses(top, n=16)
+        cams = [W.make_camera(p, yw, pt, 1920, 1080) for p, yw, pt in poses]
+        gpu.reserve(nch, 1920 * 1080)
+        depth = torch.empty((1080, 1920), dtype=torch.float32, device=dev)
+        idb = torch.empty((1080, 1920), dtype=torch.int32, device=dev)
+        out = (depth, idb, None, None)
+        gpu.splat(l0, cams[0], out=out)
+        torch.cuda.synchronize()
+        times, hits = [], []
+        for cam in cams:
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            gpu.splat(l0, cam, out=out)
+            torch.cuda.synchronize()
+            times.append(time.perf_counter() - t0)
+            hits.append(float((idb != -1).float().mean().item()))
+        fr_ms = 1e3 * sum(times) / len(times)
+        splat_res = {"splats": n0, "resolution": "1920x1080", "poses": len(cams),
+                     "ms_per_frame": round(fr_ms, 3), "splats_per_s": round(n0 / (fr_ms * 1e-3), 1),
+                     "pixels_hit": round(sum(hits) / len(hits), 3)}
+        log(f"splat: {n0} splats, {fr_ms:.2f} ms/frame -> {n0 / (fr_ms * 1e-3) / 1e9:.2f} Gsplats/s")
+
+    # ---- CPU baseline: the oracle on host cores ----
+    cpu = None
+    if not args.no_cpu:
+        from oracle import oracle as O
+
+        vh = vox.cpu().numpy()
+        mh = smap.cpu().numpy().view(np.uint16)
+        idsh = np.arange(nch, dtype=np.uint32)
+        O.remesh_time(w, vh, idsh[:64], mh, args.cpu_threads)      # warm
+        t, _ = O.remesh_time(w, vh, idsh, mh, args.cpu_threads)
+        cpu = {"value": round(nch * VOXELS_PER_CHUNK / t / 1e9, 4), "unit": "Gvoxels/s",
+               "cores": args.cpu_threads, "kind": "port",
+               "sample": f"all {nch} chunks of the same world, L0..L4 with shadows, "
+                         f"{args.cpu_threads} OpenMP threads as chunkset_manage "
+                         f"(host has {os.cpu_count()} CPUs); {t:.2f} s"}
+        log(f"cpu: {t:.2f} s -> {cpu['value']} Gvoxel/s")
+
+    graph.close()
+    line = {
+        "metric": "Gvoxels/s remeshed (batched vs per-chunk), splats/s at 1080p, % HBM peak",
+        "value": round(gvox, 3),
+        "unit": "Gvoxels/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u8",
+        "data": "synthetic (seeded value-noise terrain, 5% caves)",
+        "config": {"workload": "remesh L0-L4 of 2048x256x2048 (4096 chunks of 64^3), "
+                               "batched persistent kernel in a hipGraph",
+                   "chunks": nch, "occupancy": occupancy, "splat_records": cap,
+                   "launch": tune},
+        "per_chunk": {"gvoxels_s": round(pc_gvox, 3), "ms_per_pass": round(pc_ms, 3),
+                      "identical_output": same},
+        "speedup_vs_per_chunk": round(gvox / pc_gvox, 2),
+        "splat": splat_res,
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line))
+    gpu.close()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,187 @@
+/*
+ * vp_gpu.h -- C ABI of the MI355X (gfx950) voxel pipeline.
+ *
+ * Every entry point is extern "C", takes plain pointers and sizes, returns an
+ * int (VP_OK = 0, negative = error, never aborts) and is asynchronous on the
+ * caller's HIP stream (passed as void*; NULL = the null stream).  Device
+ * buffers belong to the caller; a vp_ctx owns only its small scratch arenas.
+ * Output order is deterministic: it never depends on scheduling.
+ *
+ * Reference interfaces replaced (paths are relative to kosshi-net/voxplat):
+ *   vp_gpu_cull_mesh  <- chunkset_manage per-chunk body   src/chunkset.c:318-505
+ *                        = chunk_make_mask                src/chunkset/mesher.c:377-456
+ *                        + chunk_make_splatlist (L0..L4)  src/chunkset/mesher.c:497-536
+ *                        + chunk_mask_downsample (L1..L4) src/chunkset/mesher.c:460-493
+ *                        + shadow_sample                  src/chunkset/shadow.h:45-63
+ *   vp_gpu_make_mask  <- chunk_make_mask                  src/chunkset/mesher.c:377-456
+ *   vp_gpu_lod_mip    <- chunk_mask_downsample            src/chunkset/mesher.c:460-493
+ *   vp_gpu_rle        <- rle_compress / rle_decompress    src/chunkset/rle.c:44-116
+ *                        (callers chunk_compress / chunk_open_*, src/chunkset.c:133-230)
+ *   vp_gpu_splat      <- gfx_svl_draw + splat.vert/.frag  src/gfx/vsplat.c:104-193,
+ *                        resources/splat.vert.glsl:34-54, resources/splat.frag.glsl:60-156
+ *   vp_gpu_splat_backward -- no counterpart in the reference (colour fitting).
+ */
+#ifndef VP_GPU_H
+#define VP_GPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+#define VP_OK               0
+#define VP_ERR_ARG         -1  /* bad argument (NULL pointer, size out of range)   */
+#define VP_ERR_CAPACITY    -2  /* an output buffer is too small (sync queries)     */
+#define VP_ERR_HIP         -3  /* a HIP runtime call failed                        */
+#define VP_ERR_UNSUPPORTED -4  /* valid in the reference but not on this path      */
+#define VP_ERR_DATA        -5  /* malformed input stream (RLE length != chunk size)*/
+#define VP_ERR_NOMEM       -6  /* scratch allocation failed                        */
+
+/* device status bits, read with vp_ctx_status() */
+#define VP_STATUS_OVERFLOW  1u  /* a splat / word list did not fit its capacity */
+#define VP_STATUS_BAD_RLE   2u  /* an RLE stream did not decode to R^3 voxels    */
+#define VP_STATUS_BAD_ID    4u  /* a listed chunk id / stream range was invalid  */
+
+typedef struct vp_ctx vp_ctx;
+
+/* World geometry, as ChunkSet::root_bitw / max_bitw (src/chunkset.h:36-40).
+ * Chunk edge R = 1 << root_bitw; chunks per axis = 1 << max_bitw[a].
+ * Voxel index inside a chunk: ((z << b | y) << b) | x   (src/chunkset.c:128-130)
+ * Chunk index:               ((cz << by | cy) << bx) | cx (src/chunkset.c:124-126)
+ * World voxels are chunk-major: chunk c occupies bytes [c*R^3, (c+1)*R^3). */
+typedef struct {
+    uint8_t root_bitw;
+    uint8_t max_bitw[3];
+} vp_world_desc;
+
+/* Shadow map (src/chunkset/shadow.h:26-63): u16 occluder heights indexed by
+ * (wx + wy) + size_x * wz, size_x = X + Y, size_z = Z.  The buffer must hold
+ * vp_shadow_padded_len() entries; entries past size_x*size_z are zero so the
+ * LOD samples that land beyond the world read "not shadowed". */
+typedef struct {
+    const uint16_t *d_map;
+    uint32_t size_x;
+    uint32_t size_z;
+} vp_shadow_desc;
+
+static inline uint64_t vp_shadow_padded_len(uint32_t size_x, uint32_t size_z)
+{
+    return (uint64_t)size_x * (uint64_t)(size_z + 17u) + 64u;
+}
+
+/* ---- context ------------------------------------------------------------ */
+int         vp_ctx_create(int device, vp_ctx **out);
+int         vp_ctx_destroy(vp_ctx *ctx);
+const char *vp_strerror(int code);
+/* Synchronises `stream`, returns and clears the device status bits. */
+int         vp_ctx_status(vp_ctx *ctx, void *stream, uint32_t *status);
+/* Number of compute units of the context's device. */
+int         vp_ctx_num_cus(vp_ctx *ctx);
+
+/* ---- cull + splat list (+ LOD) ------------------------------------------ */
+#define VP_CM_LOD        1u  /* also emit LOD levels 1..4 (chunkset.c:390-458)  */
+#define VP_CM_RECORD_V2  2u  /* v2 records instead of the reference bytes       */
+#define VP_CM_COUNT_ONLY 4u  /* write counts/offsets only, no records           */
+
+/* Splat records are 8 bytes.
+ * compat (reference bytes, mesher.c:521-531): int16 {wx, wy, wz, attr},
+ *         attr = colour | shadow << 6.
+ * v2:     uint16 {wx, wy, wz}; uint8 colour; uint8 flags
+ *         (bit a*2 = +a face exposed, bit a*2+1 = -a face exposed, bit 6 = shadow).
+ *         LOD records carry the face bits of the level-0 cell their colour
+ *         comes from.  compat == {wx, wy, wz, colour | (flags & 0x40)}. */
+typedef struct {
+    void     *d_splats;           /* capacity_records * 8 bytes                     */
+    uint64_t  capacity_records;
+    uint64_t *d_chunk_offsets;    /* n+1 exclusive prefix, in int16 items (4/splat) */
+    uint32_t *d_level_counts;     /* n*5 per-level counts, int16 items (svl_items)  */
+} vp_splat_out;
+
+/* Remesh the chunks d_chunk_ids[0..n) of a world resident in HBM.
+ * Output: the concatenation, in list order, of each chunk's L0||L1||L2||L3||L4
+ * records, each level in (z,y,x) scan order -- exactly the bytes
+ * chunkset_manage stores in ChunkMD::svl, with ChunkMD::svl_items in
+ * d_level_counts.  Chunks whose list does not fit capacity_records are not
+ * written (their counts are) and VP_STATUS_OVERFLOW is raised.
+ * d_uniform (nullable): per chunk, -1 = mixed, else the value every voxel has;
+ *   uniform chunks are not read (only their halo planes are synthesised).
+ * d_n (nullable): device-resident count <= n (for replayed graphs).
+ * n == 1 is the plain per-chunk path; n > 1 runs one persistent work-queue
+ * kernel whose output offsets come from a decoupled look-back.
+ * Supported roots on the GPU: 4..64 (root_bitw 2..6). */
+int vp_gpu_cull_mesh(vp_ctx *ctx, const vp_world_desc *world,
+                     const uint8_t *d_voxels, const int16_t *d_uniform,
+                     const vp_shadow_desc *shadow,
+                     const uint32_t *d_chunk_ids, uint32_t n, const uint32_t *d_n,
+                     uint32_t flags, const vp_splat_out *out, void *stream);
+
+/* Level-0 visibility masks ((R+1)^3 bytes per chunk, index (z*(R+1)+y)*(R+1)+x)
+ * exactly as chunk_make_mask leaves them in a zeroed buffer. */
+int vp_gpu_make_mask(vp_ctx *ctx, const vp_world_desc *world,
+                     const uint8_t *d_voxels, const int16_t *d_uniform,
+                     const uint32_t *d_chunk_ids, uint32_t n,
+                     uint8_t *d_masks, void *stream);
+
+/* One LOD step on n masks of stride (R+1)^3: d_dst = downsample(level, d_src)
+ * into a zeroed target (chunk_mask_downsample, mesher.c:460-493).  Each dst
+ * mask is fully overwritten. level in 1..4. */
+int vp_gpu_lod_mip(vp_ctx *ctx, uint32_t root_bitw, int level,
+                   const uint8_t *d_src, uint8_t *d_dst, uint32_t n, void *stream);
+
+/* ---- RLE codec ------------------------------------------------------------
+ * Word = (value << 24) | run (1..0xFFFFFF), runs maximal in voxel order,
+ * one 0 word terminates each chunk's stream (rle.c:44-87).
+ * ENCODE: chunks d_chunk_ids[0..n) of d_voxels -> d_words, concatenated in
+ *   list order with d_word_offsets[n+1] (in words, terminators included).
+ *   d_uniform_out (nullable) receives -1 / the uniform value per listed chunk.
+ * DECODE: d_words/d_word_offsets -> chunks d_chunk_ids[k] of d_voxels.
+ *   A stream that does not expand to exactly R^3 voxels raises
+ *   VP_STATUS_BAD_RLE (the chunk is written up to R^3 bytes); offsets outside
+ *   [0, capacity_words] raise VP_STATUS_BAD_ID and the chunk is skipped.
+ * Chunk ids outside the world raise VP_STATUS_BAD_ID and are skipped. */
+#define VP_RLE_ENCODE 0
+#define VP_RLE_DECODE 1
+int vp_gpu_rle(vp_ctx *ctx, int dir, const vp_world_desc *world,
+               uint8_t *d_voxels, const uint32_t *d_chunk_ids, uint32_t n,
+               uint32_t *d_words, uint64_t capacity_words,
+               uint64_t *d_word_offsets, int16_t *d_uniform_out, void *stream);
+
+/* ---- splat resolve --------------------------------------------------------
+ * Camera matrices are column-major 4x4 (m[col*4+row], cglm layout), built on
+ * the host as in src/game.c:458-491. */
+typedef struct {
+    float    view_translation[16];   /* u_view_translation                   */
+    float    proj_view_rot[16];      /* u_projection * u_view_rotation       */
+    float    inv_proj_view_rot[16];  /* u_inv_proj_view_rot                  */
+    uint32_t width, height;          /* uViewport                            */
+    float    far_plane;              /* u_far                                */
+} vp_camera;
+
+typedef struct {
+    float    *d_depth;   /* W*H, t0/far of the nearest hit, 1.0 = none          */
+    uint32_t *d_id;      /* W*H, splat index of the nearest hit, ~0u = none     */
+    float    *d_rgb;     /* nullable W*H*3: shaded + fogged colour              */
+    float    *d_weight;  /* nullable W*H: d rgb / d voxel_rgb (shade*(1-fog))    */
+} vp_frame;
+
+/* d_splats: n compat records; lod per splat = 1 << d_lod[i] (nullable: lod_log2
+ * for all). d_voxel_rgb (nullable): n*3 fp32 colours that replace the 6-bit
+ * palette (the fitting path). Pixel (px,py) has gl_FragCoord (px+.5, py+.5),
+ * row 0 at the bottom, as GL. Ties go to the lower splat index. */
+int vp_gpu_splat(vp_ctx *ctx, const int16_t *d_splats, const uint8_t *d_lod,
+                 uint32_t lod_log2, uint64_t n, const vp_camera *cam,
+                 const float *d_voxel_rgb, const vp_frame *out, void *stream);
+
+/* dL/d voxel_rgb += sum over pixels p with id(p)=i of weight(p) * dL/drgb(p).
+ * d_id/d_weight are the forward's outputs; d_grad (n*3) is accumulated. */
+int vp_gpu_splat_backward(vp_ctx *ctx, uint64_t n, const vp_camera *cam,
+                          const uint32_t *d_id, const float *d_weight,
+                          const float *d_dL_drgb, float *d_grad, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* VP_GPU_H */

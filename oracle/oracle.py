@@ -1,0 +1,225 @@
+"""ctypes wrapper of the CPU oracle (libvp_oracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, as the checker and the CPU baseline.  The
+product (voxplat_amd/) never imports it.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import pathlib
+
+import numpy as np
+
+from voxplat_amd.abi import Camera, GenParams, WorldDesc, shadow_padded_len
+
+_HERE = pathlib.Path(__file__).resolve().parent
+LIB_PATH = _HERE / "libvp_oracle.so"
+
+
+_P = C.c_void_p
+_lib = None
+
+
+def lib() -> C.CDLL:
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB_PATH.exists():
+        from . import build_oracle
+        build_oracle.build()
+    L = C.CDLL(str(LIB_PATH))
+    W = C.POINTER(WorldDesc)
+    G = C.POINTER(GenParams)
+    sig = {
+        "vpo_splitmix64": (C.c_uint64, [C.c_uint64]),
+        "vpo_gen_height": (C.c_int32, [G, C.c_uint32, C.c_uint32]),
+        "vpo_gen_chunk": (None, [W, G, C.c_uint32, _P]),
+        "vpo_gen_world": (None, [W, G, _P, C.c_int]),
+        "vpo_shadow_dims": (None, [W, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
+        "vpo_shadow_sample": (C.c_int, [_P, C.c_uint32, _P]),
+        "vpo_shadow_place": (None, [_P, C.c_uint32, _P]),
+        "vpo_shadow_build": (None, [W, _P, _P, C.c_int, C.c_int]),
+        "vpo_make_mask": (None, [W, _P, C.c_uint32, _P]),
+        "vpo_mask_downsample": (None, [C.c_uint32, C.c_int, _P, _P]),
+        "vpo_make_splatlist": (C.c_uint32, [W, C.c_uint32, C.c_int, _P, _P, C.c_uint32, _P]),
+        "vpo_chunk_remesh": (C.c_uint32, [W, _P, C.c_uint32, _P, C.c_uint32, C.c_int, _P, _P,
+                                          _P, _P]),
+        "vpo_remesh_list": (C.c_int64, [W, _P, _P, C.c_uint32, _P, C.c_uint32, C.c_int, C.c_int,
+                                        _P, C.c_uint64, _P, _P]),
+        "vpo_remesh_time": (C.c_double, [W, _P, _P, C.c_uint32, _P, C.c_uint32, C.c_int, _P]),
+        "vpo_rle_compress": (C.c_uint32, [_P, C.c_uint32, _P, C.c_uint32]),
+        "vpo_rle_decompress": (C.c_int64, [_P, C.c_uint32, _P, C.c_uint32]),
+        "vpo_edit_sphere": (None, [W, _P, _P, C.c_uint32, _P, C.c_uint32, C.c_uint8, _P]),
+        "vpo_splat_render": (None, [_P, _P, C.c_uint32, C.c_uint64, C.POINTER(Camera), _P, _P,
+                                    _P, _P, _P]),
+        "vpo_splat_backward": (None, [C.c_uint64, C.POINTER(Camera), _P, _P, _P, _P]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = L
+    return L
+
+
+def _p(a: np.ndarray | None):
+    if a is None:
+        return None
+    assert a.flags.c_contiguous
+    return a.ctypes.data
+
+
+def gen_params(seed: int, period: int, hmin: int, hmax: int) -> GenParams:
+    return GenParams(seed, period, hmin, hmax)
+
+
+def gen_height(p: GenParams, x: int, z: int) -> int:
+    return lib().vpo_gen_height(C.byref(p), x, z)
+
+
+def gen_chunk(w: WorldDesc, p: GenParams, chunk: int) -> np.ndarray:
+    out = np.empty(w.R ** 3, np.uint8)
+    lib().vpo_gen_chunk(C.byref(w), C.byref(p), chunk, _p(out))
+    return out
+
+
+def gen_world(w: WorldDesc, p: GenParams, threads: int = 8) -> np.ndarray:
+    out = np.empty(w.chunks * w.R ** 3, np.uint8)
+    lib().vpo_gen_world(C.byref(w), C.byref(p), _p(out), threads)
+    return out
+
+
+def shadow_build(w: WorldDesc, voxels: np.ndarray, serial: bool = False,
+                 threads: int = 8) -> np.ndarray:
+    sx, sz = w.shadow_dims()
+    m = np.empty(shadow_padded_len(sx, sz), np.uint16)
+    lib().vpo_shadow_build(C.byref(w), _p(voxels), _p(m), int(serial), threads)
+    return m
+
+
+def shadow_sample(m: np.ndarray, sx: int, ws) -> int:
+    a = np.asarray(ws, np.uint32)
+    return lib().vpo_shadow_sample(_p(m), sx, _p(a))
+
+
+def shadow_place(m: np.ndarray, sx: int, ws) -> None:
+    a = np.asarray(ws, np.uint32)
+    lib().vpo_shadow_place(_p(m), sx, _p(a))
+
+
+def make_mask(w: WorldDesc, voxels: np.ndarray, chunk: int) -> np.ndarray:
+    n = w.R + 1
+    m = np.zeros(n ** 3, np.uint8)
+    lib().vpo_make_mask(C.byref(w), _p(voxels), chunk, _p(m))
+    return m
+
+
+def mask_downsample(R: int, level: int, src: np.ndarray) -> np.ndarray:
+    dst = np.zeros_like(src)
+    lib().vpo_mask_downsample(R, level, _p(src), _p(dst))
+    return dst
+
+
+def make_splatlist(w: WorldDesc, chunk: int, level: int, mask: np.ndarray,
+                   smap: np.ndarray | None) -> np.ndarray:
+    sx, _ = w.shadow_dims()
+    n = (w.R >> level) + 1
+    g = np.empty(4 * n ** 3, np.int16)
+    if smap is None:
+        smap = np.zeros(shadow_padded_len(*w.shadow_dims()), np.uint16)
+    k = lib().vpo_make_splatlist(C.byref(w), chunk, level, _p(mask), _p(smap), sx, _p(g))
+    return g[:k].copy()
+
+
+def chunk_remesh(w: WorldDesc, voxels: np.ndarray, chunk: int, smap: np.ndarray,
+                 lod: bool = True):
+    sx, _ = w.shadow_dims()
+    n = (w.R + 1) ** 3
+    geom = np.empty(4 * sum(((w.R >> l) + 1) ** 3 for l in range(5)), np.int16)
+    items = np.zeros(5, np.uint32)
+    mask = np.empty(n, np.uint8)
+    work = np.empty(n, np.uint8)
+    t = lib().vpo_chunk_remesh(C.byref(w), _p(voxels), chunk, _p(smap), sx, int(lod), _p(geom),
+                               _p(items), _p(mask), _p(work))
+    return geom[:t].copy(), items
+
+
+def remesh_list(w: WorldDesc, voxels: np.ndarray, ids: np.ndarray, smap: np.ndarray,
+                lod: bool = True, threads: int = 8):
+    """Returns (records int16[N,4], offsets u64[n+1] in int16 items, counts u32[n,5])."""
+    ids = np.ascontiguousarray(ids, np.uint32)
+    n = len(ids)
+    sx, _ = w.shadow_dims()
+    offsets = np.zeros(n + 1, np.uint64)
+    counts = np.zeros((n, 5), np.uint32)
+    # sizing pass
+    lib().vpo_remesh_list(C.byref(w), _p(voxels), _p(ids), n, _p(smap), sx, int(lod), threads,
+                          None, 0, _p(offsets), _p(counts))
+    total = int(offsets[n])
+    out = np.empty(max(total, 4), np.int16)
+    r = lib().vpo_remesh_list(C.byref(w), _p(voxels), _p(ids), n, _p(smap), sx, int(lod),
+                              threads, _p(out), total, _p(offsets), _p(counts))
+    assert r == total
+    return out[:total].reshape(-1, 4), offsets, counts
+
+
+def remesh_time(w: WorldDesc, voxels: np.ndarray, ids: np.ndarray, smap: np.ndarray,
+                threads: int):
+    ids = np.ascontiguousarray(ids, np.uint32)
+    sx, _ = w.shadow_dims()
+    tot = C.c_uint64()
+    t = lib().vpo_remesh_time(C.byref(w), _p(voxels), _p(ids), len(ids), _p(smap), sx, threads,
+                              C.byref(tot))
+    return t, tot.value
+
+
+def rle_compress(data: np.ndarray) -> np.ndarray:
+    data = np.ascontiguousarray(data, np.uint8)
+    cap = len(data) + 2
+    w = np.empty(cap, np.uint32)
+    k = lib().vpo_rle_compress(_p(data), len(data), _p(w), cap)
+    assert k > 0
+    return w[:k].copy()
+
+
+def rle_decompress(words: np.ndarray, cap: int):
+    words = np.ascontiguousarray(words, np.uint32)
+    out = np.zeros(cap, np.uint8)
+    n = lib().vpo_rle_decompress(_p(words), len(words), _p(out), cap)
+    return out, n
+
+
+def edit_sphere(w: WorldDesc, voxels: np.ndarray, smap: np.ndarray | None, centre, radius: int,
+                voxel: int, dirty: np.ndarray | None = None) -> None:
+    sx, _ = w.shadow_dims()
+    c = np.asarray(centre, np.int32)
+    lib().vpo_edit_sphere(C.byref(w), _p(voxels), _p(smap), sx, _p(c), radius, voxel, _p(dirty))
+
+
+def splat_render(splats: np.ndarray, cam: Camera, lod: np.ndarray | None = None,
+                 lod_log2: int = 0, voxel_rgb: np.ndarray | None = None, rgb: bool = False,
+                 weight: bool = False):
+    splats = np.ascontiguousarray(splats, np.int16).reshape(-1, 4)
+    n = len(splats)
+    H, W = cam.height, cam.width
+    depth = np.empty((H, W), np.float32)
+    ids = np.empty((H, W), np.uint32)
+    col = np.empty((H, W, 3), np.float32) if rgb else None
+    wt = np.empty((H, W), np.float32) if weight else None
+    if voxel_rgb is not None:
+        voxel_rgb = np.ascontiguousarray(voxel_rgb, np.float32)
+    if lod is not None:
+        lod = np.ascontiguousarray(lod, np.uint8)
+    lib().vpo_splat_render(_p(splats), _p(lod), lod_log2, n, C.byref(cam), _p(voxel_rgb),
+                           _p(depth), _p(ids), _p(col), _p(wt))
+    return depth, ids, col, wt
+
+
+def splat_backward(n: int, cam: Camera, ids: np.ndarray, weight: np.ndarray,
+                   dL: np.ndarray) -> np.ndarray:
+    g = np.zeros((n, 3), np.float64)
+    lib().vpo_splat_backward(n, C.byref(cam), _p(np.ascontiguousarray(ids, np.uint32)),
+                             _p(np.ascontiguousarray(weight, np.float32)),
+                             _p(np.ascontiguousarray(dL, np.float32)), _p(g))
+    return g

@@ -1,0 +1,10 @@
+"""voxplat_amd: MI355X (gfx950) voxel pipeline -- remesh (cull + splat list +
+LOD), RLE codec and splat resolve as HIP kernels behind the C ABI of
+include/vp_gpu.h.
+
+    from voxplat_amd import GPU, abi, workload
+"""
+from . import abi, workload
+from ._lib import GPU, RemeshGraph, VPError, host_shadow_spheres, load_library
+
+__all__ = ["GPU", "RemeshGraph", "VPError", "abi", "workload", "load_library", "host_shadow_spheres"]

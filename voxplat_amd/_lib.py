@@ -1,0 +1,350 @@
+"""Loader and thin torch-facing wrapper of libvp_gpu.so (the C ABI).
+
+The HIP library is the only implementation: if it is missing this module
+raises instead of falling back to anything else.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import pathlib
+
+from . import abi
+
+_PKG = pathlib.Path(__file__).resolve().parent
+LIB_PATH = pathlib.Path(os.environ.get("VP_GPU_LIB", _PKG / "libvp_gpu.so"))
+
+_lib = None
+
+
+def load_library() -> C.CDLL:
+    """Load libvp_gpu.so and bind every prototype of include/vp_gpu.h."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB_PATH.exists():
+        raise ImportError(
+            f"{LIB_PATH} is missing: build it with `python -m voxplat_amd.build` "
+            "(there is no CPU fallback)")
+    lib = C.CDLL(str(LIB_PATH))
+    for name, (res, args) in abi.PROTOTYPES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+class VPError(RuntimeError):
+    def __init__(self, code: int, what: str):
+        self.code = code
+        msg = load_library().vp_strerror(code).decode()
+        super().__init__(f"{what}: {msg} ({code})")
+
+
+def _check(code: int, what: str) -> None:
+    if code != abi.VP_OK:
+        raise VPError(code, what)
+
+
+def _ptr(t) -> int | None:
+    if t is None:
+        return None
+    return t.data_ptr()
+
+
+class GPU:
+    """One vp_ctx on a device, with helpers that take torch tensors."""
+
+    def __init__(self, device: int = 0):
+        import torch
+
+        self.torch = torch
+        self.lib = load_library()
+        self.device = device
+        torch.cuda.set_device(device)
+        h = C.c_void_p()
+        _check(self.lib.vp_ctx_create(device, C.byref(h)), "vp_ctx_create")
+        self.ctx = h
+
+    def close(self) -> None:
+        if self.ctx:
+            self.lib.vp_ctx_destroy(self.ctx)
+            self.ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _stream(self, stream=None) -> int:
+        s = stream if stream is not None else self.torch.cuda.current_stream(self.device)
+        return s.cuda_stream
+
+    def status(self, stream=None) -> int:
+        v = C.c_uint32()
+        _check(self.lib.vp_ctx_status(self.ctx, C.c_void_p(self._stream(stream)), C.byref(v)),
+               "vp_ctx_status")
+        return v.value
+
+    @property
+    def num_cus(self) -> int:
+        return self.lib.vp_ctx_num_cus(self.ctx)
+
+    # ---- remesh --------------------------------------------------------------
+    def cull_mesh(self, world: abi.WorldDesc, voxels, ids, shadow_map=None, uniform=None,
+                  lod: bool = True, v2: bool = False, capacity: int | None = None,
+                  count_only: bool = False, d_n=None, out=None, stream=None):
+        """Remesh chunks `ids` (int32 tensor). Returns (records int16[N,4] or None,
+        offsets int64[n+1] in int16 items, level_counts int32[n,5])."""
+        torch = self.torch
+        n = int(ids.numel())
+        dev = voxels.device
+        flags = (abi.VP_CM_LOD if lod else 0) | (abi.VP_CM_RECORD_V2 if v2 else 0)
+        sh = abi.ShadowDesc()
+        if shadow_map is not None:
+            sx, sz = world.shadow_dims()
+            sh.d_map, sh.size_x, sh.size_z = shadow_map.data_ptr(), sx, sz
+        st = C.c_void_p(self._stream(stream))
+        if out is None:
+            offsets = torch.empty(n + 1, dtype=torch.int64, device=dev)
+            counts = torch.empty((n, 5), dtype=torch.int32, device=dev)
+        else:
+            offsets, counts = out
+        so = abi.SplatOut()
+        so.d_chunk_offsets, so.d_level_counts = offsets.data_ptr(), counts.data_ptr()
+        if count_only or capacity is None:
+            so.d_splats, so.capacity_records = None, 0
+            _check(self.lib.vp_gpu_cull_mesh(
+                self.ctx, C.byref(world), C.c_void_p(voxels.data_ptr()),
+                C.c_void_p(_ptr(uniform)), C.byref(sh), C.c_void_p(ids.data_ptr()), n,
+                C.c_void_p(_ptr(d_n)), flags | abi.VP_CM_COUNT_ONLY, C.byref(so), st),
+                "vp_gpu_cull_mesh(count)")
+            if count_only:
+                return None, offsets, counts
+            capacity = int(offsets[n].item()) // 4
+        recs = torch.empty((max(capacity, 1), 4), dtype=torch.int16, device=dev)
+        so.d_splats, so.capacity_records = recs.data_ptr(), capacity
+        _check(self.lib.vp_gpu_cull_mesh(
+            self.ctx, C.byref(world), C.c_void_p(voxels.data_ptr()), C.c_void_p(_ptr(uniform)),
+            C.byref(sh), C.c_void_p(ids.data_ptr()), n, C.c_void_p(_ptr(d_n)), flags,
+            C.byref(so), st), "vp_gpu_cull_mesh")
+        return recs, offsets, counts
+
+    def make_mask(self, world: abi.WorldDesc, voxels, ids, uniform=None, stream=None):
+        torch = self.torch
+        n = int(ids.numel())
+        N0 = world.R + 1
+        masks = torch.empty((n, N0, N0, N0), dtype=torch.uint8, device=voxels.device)
+        _check(self.lib.vp_gpu_make_mask(
+            self.ctx, C.byref(world), C.c_void_p(voxels.data_ptr()), C.c_void_p(_ptr(uniform)),
+            C.c_void_p(ids.data_ptr()), n, C.c_void_p(masks.data_ptr()),
+            C.c_void_p(self._stream(stream))), "vp_gpu_make_mask")
+        return masks
+
+    def lod_mip(self, root_bitw: int, level: int, src, stream=None):
+        dst = self.torch.empty_like(src)
+        n = src.shape[0]
+        _check(self.lib.vp_gpu_lod_mip(self.ctx, root_bitw, level, C.c_void_p(src.data_ptr()),
+                                       C.c_void_p(dst.data_ptr()), n,
+                                       C.c_void_p(self._stream(stream))), "vp_gpu_lod_mip")
+        return dst
+
+    # ---- RLE -----------------------------------------------------------------
+    def rle_encode(self, world: abi.WorldDesc, voxels, ids, capacity: int, uniform_out=True,
+                   stream=None):
+        torch = self.torch
+        n = int(ids.numel())
+        dev = voxels.device
+        words = torch.empty(max(capacity, 1), dtype=torch.int32, device=dev)
+        offsets = torch.empty(n + 1, dtype=torch.int64, device=dev)
+        uni = torch.empty(n, dtype=torch.int16, device=dev) if uniform_out else None
+        _check(self.lib.vp_gpu_rle(self.ctx, abi.VP_RLE_ENCODE, C.byref(world),
+                                   C.c_void_p(voxels.data_ptr()), C.c_void_p(ids.data_ptr()), n,
+                                   C.c_void_p(words.data_ptr()), capacity,
+                                   C.c_void_p(offsets.data_ptr()), C.c_void_p(_ptr(uni)),
+                                   C.c_void_p(self._stream(stream))), "vp_gpu_rle(encode)")
+        return words, offsets, uni
+
+    def rle_decode(self, world: abi.WorldDesc, words, offsets, ids, voxels, stream=None):
+        n = int(ids.numel())
+        _check(self.lib.vp_gpu_rle(self.ctx, abi.VP_RLE_DECODE, C.byref(world),
+                                   C.c_void_p(voxels.data_ptr()), C.c_void_p(ids.data_ptr()), n,
+                                   C.c_void_p(words.data_ptr()), int(words.numel()),
+                                   C.c_void_p(offsets.data_ptr()), None,
+                                   C.c_void_p(self._stream(stream))), "vp_gpu_rle(decode)")
+        return voxels
+
+    # ---- splat -----------------------------------------------------------------
+    def splat(self, splats, cam: abi.Camera, lod=None, lod_log2: int = 0, voxel_rgb=None,
+              rgb: bool = False, weight: bool = False, out=None, stream=None):
+        torch = self.torch
+        W, H = cam.width, cam.height
+        dev = splats.device
+        n = int(splats.shape[0])
+        if out is None:
+            depth = torch.empty((H, W), dtype=torch.float32, device=dev)
+            ids = torch.empty((H, W), dtype=torch.int32, device=dev)
+            col = torch.empty((H, W, 3), dtype=torch.float32, device=dev) if rgb else None
+            wt = torch.empty((H, W), dtype=torch.float32, device=dev) if weight else None
+        else:
+            depth, ids, col, wt = out
+        fr = abi.Frame()
+        fr.d_depth, fr.d_id, fr.d_rgb, fr.d_weight = (depth.data_ptr(), ids.data_ptr(),
+                                                      _ptr(col), _ptr(wt))
+        _check(self.lib.vp_gpu_splat(self.ctx, C.c_void_p(splats.data_ptr()),
+                                     C.c_void_p(_ptr(lod)), lod_log2, n, C.byref(cam),
+                                     C.c_void_p(_ptr(voxel_rgb)), C.byref(fr),
+                                     C.c_void_p(self._stream(stream))), "vp_gpu_splat")
+        return depth, ids, col, wt
+
+    def splat_backward(self, n: int, cam: abi.Camera, ids, weight, dL_drgb, grad, stream=None):
+        _check(self.lib.vp_gpu_splat_backward(self.ctx, n, C.byref(cam),
+                                              C.c_void_p(ids.data_ptr()),
+                                              C.c_void_p(weight.data_ptr()),
+                                              C.c_void_p(dL_drgb.data_ptr()),
+                                              C.c_void_p(grad.data_ptr()),
+                                              C.c_void_p(self._stream(stream))),
+               "vp_gpu_splat_backward")
+        return grad
+
+    # ---- workload builders (include/vp_workload.h) -------------------------------
+    def generate(self, world: abi.WorldDesc, params: abi.GenParams, voxels, uniform=None,
+                 chunk0: int = 0, count: int | None = None, stream=None):
+        count = world.chunks - chunk0 if count is None else count
+        _check(self.lib.vp_gpu_generate(self.ctx, C.byref(world), C.byref(params),
+                                        C.c_void_p(voxels.data_ptr()), chunk0, count,
+                                        C.c_void_p(_ptr(uniform)),
+                                        C.c_void_p(self._stream(stream))), "vp_gpu_generate")
+
+    def classify(self, world: abi.WorldDesc, voxels, uniform, chunk0: int = 0,
+                 count: int | None = None, stream=None):
+        count = world.chunks - chunk0 if count is None else count
+        _check(self.lib.vp_gpu_classify(self.ctx, C.byref(world), C.c_void_p(voxels.data_ptr()),
+                                        chunk0, count, C.c_void_p(uniform.data_ptr()),
+                                        C.c_void_p(self._stream(stream))), "vp_gpu_classify")
+
+    def shadow_build(self, world: abi.WorldDesc, voxels, smap, stream=None):
+        _check(self.lib.vp_gpu_shadow_build(self.ctx, C.byref(world),
+                                            C.c_void_p(voxels.data_ptr()),
+                                            C.c_void_p(smap.data_ptr()),
+                                            C.c_void_p(self._stream(stream))),
+               "vp_gpu_shadow_build")
+
+    def edit_spheres(self, world: abi.WorldDesc, voxels, spheres, dirty=None, uniform=None,
+                     stream=None):
+        arr = (abi.Sphere * len(spheres))(*spheres)
+        _check(self.lib.vp_gpu_edit_spheres(self.ctx, C.byref(world),
+                                            C.c_void_p(voxels.data_ptr()), arr, len(spheres),
+                                            C.c_void_p(_ptr(dirty)), C.c_void_p(_ptr(uniform)),
+                                            C.c_void_p(self._stream(stream))),
+               "vp_gpu_edit_spheres")
+
+
+def host_shadow_spheres(world: abi.WorldDesc, smap_host, spheres) -> None:
+    """Sequential shadow_place_update of add-sphere voxels on a host map (numpy u16)."""
+    lib = load_library()
+    arr = (abi.Sphere * len(spheres))(*spheres)
+    _check(lib.vp_host_shadow_spheres(C.byref(world), C.c_void_p(smap_host.ctypes.data), arr,
+                                      len(spheres)), "vp_host_shadow_spheres")
+
+
+class RemeshGraph:
+    """The batched remesh of one frame captured in a hipGraph (vp_host.h)."""
+
+    def __init__(self, gpu: GPU, world: abi.WorldDesc, voxels, ids, shadow_map, offsets, counts,
+                 splats, capacity: int, uniform=None, d_n=None, lod: bool = True,
+                 v2: bool = False):
+        self.gpu = gpu
+        self._keep = (world, voxels, ids, shadow_map, offsets, counts, splats, uniform, d_n)
+        sh = abi.ShadowDesc()
+        if shadow_map is not None:
+            sx, sz = world.shadow_dims()
+            sh.d_map, sh.size_x, sh.size_z = shadow_map.data_ptr(), sx, sz
+        so = abi.SplatOut()
+        so.d_splats, so.capacity_records = splats.data_ptr(), capacity
+        so.d_chunk_offsets, so.d_level_counts = offsets.data_ptr(), counts.data_ptr()
+        self._sh, self._so = sh, so
+        flags = (abi.VP_CM_LOD if lod else 0) | (abi.VP_CM_RECORD_V2 if v2 else 0)
+        h = C.c_void_p()
+        _check(gpu.lib.vp_host_remesh_graph_create(
+            gpu.ctx, C.byref(world), C.c_void_p(voxels.data_ptr()), C.c_void_p(_ptr(uniform)),
+            C.byref(sh), C.c_void_p(ids.data_ptr()), int(ids.numel()), C.c_void_p(_ptr(d_n)),
+            flags, C.byref(so), C.byref(h)), "vp_host_remesh_graph_create")
+        self.h = h
+
+    def launch(self, stream=None):
+        _check(self.gpu.lib.vp_host_remesh_graph_launch(self.h, C.c_void_p(self.gpu._stream(stream))),
+               "vp_host_remesh_graph_launch")
+
+    def close(self):
+        if self.h:
+            self.gpu.lib.vp_host_remesh_graph_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def _gpu_reserve(self: GPU, max_chunks: int, max_pixels: int = 0) -> None:
+    _check(self.lib.vp_ctx_reserve(self.ctx, max_chunks, max_pixels), "vp_ctx_reserve")
+
+
+def _gpu_set_launch(self: GPU, block: int, wg_per_cu: int) -> None:
+    _check(self.lib.vp_ctx_set_launch(self.ctx, block, wg_per_cu), "vp_ctx_set_launch")
+
+
+def _gpu_get_launch(self: GPU):
+    b, w = C.c_int(), C.c_int()
+    _check(self.lib.vp_ctx_get_launch(self.ctx, C.byref(b), C.byref(w)), "vp_ctx_get_launch")
+    return b.value, w.value
+
+
+def _gpu_remesh_per_chunk(self: GPU, world: abi.WorldDesc, voxels, ids, shadow_map, splats,
+                          rec_offsets, rec_caps, offsets2, counts, uniform=None, lod=True,
+                          stream=None):
+    """The plain path: one launch per chunk, driven by the C host loop."""
+    import numpy as np
+
+    sh = abi.ShadowDesc()
+    if shadow_map is not None:
+        sx, sz = world.shadow_dims()
+        sh.d_map, sh.size_x, sh.size_z = shadow_map.data_ptr(), sx, sz
+    ro = np.ascontiguousarray(rec_offsets, np.uint64)
+    rc = np.ascontiguousarray(rec_caps, np.uint64)
+    flags = abi.VP_CM_LOD if lod else 0
+    _check(self.lib.vp_host_remesh_per_chunk(
+        self.ctx, C.byref(world), C.c_void_p(voxels.data_ptr()), C.c_void_p(_ptr(uniform)),
+        C.byref(sh), C.c_void_p(ids.data_ptr()), int(ids.numel()), flags,
+        C.c_void_p(splats.data_ptr()), C.c_void_p(ro.ctypes.data), C.c_void_p(rc.ctypes.data),
+        C.c_void_p(offsets2.data_ptr()), C.c_void_p(counts.data_ptr()),
+        C.c_void_p(self._stream(stream))), "vp_host_remesh_per_chunk")
+
+
+def _gpu_autotune(self: GPU, world: abi.WorldDesc, voxels, ids, shadow_map, splats, capacity,
+                  offsets, counts, uniform=None, reps: int = 5, lod=True, stream=None):
+    sh = abi.ShadowDesc()
+    if shadow_map is not None:
+        sx, sz = world.shadow_dims()
+        sh.d_map, sh.size_x, sh.size_z = shadow_map.data_ptr(), sx, sz
+    so = abi.SplatOut()
+    so.d_splats, so.capacity_records = splats.data_ptr(), capacity
+    so.d_chunk_offsets, so.d_level_counts = offsets.data_ptr(), counts.data_ptr()
+    b, w, ms = C.c_int(), C.c_int(), C.c_float()
+    _check(self.lib.vp_host_autotune_remesh(
+        self.ctx, C.byref(world), C.c_void_p(voxels.data_ptr()), C.c_void_p(_ptr(uniform)),
+        C.byref(sh), C.c_void_p(ids.data_ptr()), int(ids.numel()),
+        abi.VP_CM_LOD if lod else 0, C.byref(so), reps, C.c_void_p(self._stream(stream)),
+        C.byref(b), C.byref(w), C.byref(ms)), "vp_host_autotune_remesh")
+    return b.value, w.value, ms.value
+
+
+GPU.reserve = _gpu_reserve
+GPU.set_launch = _gpu_set_launch
+GPU.get_launch = _gpu_get_launch
+GPU.remesh_per_chunk = _gpu_remesh_per_chunk
+GPU.autotune_remesh = _gpu_autotune

@@ -1,0 +1,410 @@
+// rle.hip -- chunk RLE encode/decode for gfx950.
+//
+// Replaces rle_compress / rle_decompress (src/chunkset/rle.c:44-116) and the
+// per-chunk (de)compression done by chunk_compress / chunk_open_*
+// (src/chunkset.c:133-230).  Word format: (value << 24) | run, runs maximal
+// in voxel order, one 0 word after each chunk's runs.
+//
+// Encode, one workgroup per chunk (persistent, ticket order = list order):
+//  1. 16-byte loads; a run starts where a byte differs from the byte before it
+//     -> one start bit per voxel in an LDS bitmap (R^3 bits = 32 KiB at R=64).
+//  2. popcounts per 64-bit bitmap word -> workgroup scan -> word offsets; a
+//     reverse scan gives, per bitmap word, the next start after it, so every
+//     run's length is local arithmetic.
+//  3. the chunk's word count is published and its output offset taken by a
+//     decoupled look-back (same scheme as the remesh kernel).
+//  4. each thread writes the words of its bitmap words.
+// Decode, one workgroup per chunk: run lengths are scanned into start
+// positions, set as bits of an LDS bitmap; an output byte's run is the rank of
+// the last start at or before it, so every 16-byte store is independent.
+#include "vp_internal.h"
+
+namespace vp {
+
+constexpr uint64_t kRFlagAgg = 1ull << 62;
+constexpr uint64_t kRFlagPre = 2ull << 62;
+constexpr uint64_t kRValMask = (1ull << 62) - 1;
+
+struct RleArgs {
+    uint8_t        *voxels;
+    const uint32_t *ids;
+    uint32_t        n;
+    uint32_t       *words;
+    uint64_t        cap;
+    uint64_t       *offsets;
+    int16_t        *uniform_out;
+    uint32_t       *ticket;
+    uint64_t       *state;
+    uint32_t       *status;
+    uint8_t         single;
+    uint32_t        nchunks;
+};
+
+__device__ __forceinline__ uint32_t r_lane() { return threadIdx.x & 63u; }
+
+// start bits of the 4 bytes of w given the byte before it
+__device__ __forceinline__ uint32_t diff4(uint32_t w, uint32_t prev)
+{
+    const uint32_t shifted = (w << 8) | (prev & 0xFFu);
+    const uint32_t d = w ^ shifted;
+    const uint32_t m = (((d & 0x7F7F7F7Fu) + 0x7F7F7F7Fu) | d) & 0x80808080u;
+    return (((m >> 7) * 0x00204081u) >> 21) & 0xFu;
+}
+
+template <int BS>
+__device__ __forceinline__ uint32_t r_scan_excl(uint32_t v, uint32_t *wsum, uint32_t &total)
+{
+    constexpr int NW = BS / 64;
+    const uint32_t lane = r_lane(), w = threadIdx.x >> 6;
+    uint32_t x = v;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        uint32_t y = __shfl_up(x, d, 64);
+        if (lane >= (uint32_t)d) x += y;
+    }
+    if (lane == 63) wsum[w] = x;
+    __syncthreads();
+    if (w == 0) {
+        uint32_t s = lane < (uint32_t)NW ? wsum[lane] : 0u, t = s;
+#pragma unroll
+        for (int d = 1; d < NW; d <<= 1) {
+            uint32_t y = __shfl_up(t, d, 64);
+            if (lane >= (uint32_t)d) t += y;
+        }
+        if (lane < (uint32_t)NW) wsum[lane] = t - s;
+        if (lane == (uint32_t)NW - 1) wsum[NW] = t;
+    }
+    __syncthreads();
+    total = wsum[NW];
+    const uint32_t r = wsum[w] + x - v;
+    __syncthreads();
+    return r;
+}
+
+// suffix-min over the block (thread order): returns min of v over threads > t
+template <int BS>
+__device__ __forceinline__ uint32_t r_suffix_min_excl(uint32_t v, uint32_t *wtmp)
+{
+    constexpr int NW = BS / 64;
+    const uint32_t lane = r_lane(), w = threadIdx.x >> 6;
+    uint32_t x = v;   // inclusive suffix min inside the wave
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        uint32_t y = __shfl_down(x, d, 64);
+        if (lane + d < 64u) x = min(x, y);
+    }
+    if (lane == 0) wtmp[w] = x;
+    __syncthreads();
+    uint32_t after = 0xFFFFFFFFu;   // min over later waves
+    for (int j = (int)w + 1; j < NW; j++) after = min(after, wtmp[j]);
+    const uint32_t nxt = __shfl_down(x, 1, 64);
+    const uint32_t excl = (lane < 63u) ? min(nxt, after) : after;
+    __syncthreads();
+    return excl;
+}
+
+template <int B, int BS>
+__device__ void rle_encode_chunk(const RleArgs &a, uint32_t k, uint32_t n, uint64_t *bm,
+                                 uint32_t *woff, uint32_t *wnext, uint32_t *wsum,
+                                 uint64_t *s_base)
+{
+    constexpr int R = 1 << B, VOX = R * R * R, WORDS = VOX / 64, WPT = (WORDS + BS - 1) / BS;
+    uint32_t chunk = a.ids[k];
+    if (chunk >= a.nchunks) {       // encode the all-air null chunk instead
+        if (threadIdx.x == 0) atomicOr(a.status, (uint32_t)VP_STATUS_BAD_ID);
+        chunk = 0xFFFFFFFFu;
+    }
+    const uint8_t *src = chunk == 0xFFFFFFFFu ? nullptr : a.voxels + ((size_t)chunk << (3 * B));
+
+    // ---- 1. start bits ----
+    if constexpr (VOX >= 16) {
+        uint16_t *b16 = reinterpret_cast<uint16_t *>(bm);
+        for (int s = threadIdx.x; s < VOX / 16; s += BS) {
+            if (!src) {
+                b16[s] = s ? 0 : 1;
+                continue;
+            }
+            const uint4 v = *reinterpret_cast<const uint4 *>(src + (size_t)s * 16);
+            const uint32_t prev = s ? src[(size_t)s * 16 - 1] : (uint32_t)(~v.x & 0xFFu);
+            uint32_t m = diff4(v.x, prev);
+            m |= diff4(v.y, v.x >> 24) << 4;
+            m |= diff4(v.z, v.y >> 24) << 8;
+            m |= diff4(v.w, v.z >> 24) << 12;
+            b16[s] = (uint16_t)m;
+        }
+    }
+    __syncthreads();
+
+    // ---- 2. per-thread bitmap words: counts, offsets, next start ----
+    uint32_t mine = 0, first = 0xFFFFFFFFu;
+    const int w0 = (int)threadIdx.x * WPT;
+    for (int j = 0; j < WPT; j++) {
+        const int w = w0 + j;
+        if (w >= WORDS) break;
+        const uint64_t b = bm[w];
+        mine += (uint32_t)__popcll(b);
+        if (b && first == 0xFFFFFFFFu) first = (uint32_t)(w * 64 + __builtin_ctzll(b));
+    }
+    uint32_t runs = 0;
+    uint32_t run = r_scan_excl<BS>(mine, wsum, runs);
+    uint32_t nxt = r_suffix_min_excl<BS>(first, wsum);
+    if (nxt == 0xFFFFFFFFu) nxt = VOX;
+    // walk own words backwards to give each word the first start after it
+    for (int j = WPT - 1; j >= 0; j--) {
+        const int w = w0 + j;
+        if (w >= WORDS) continue;
+        wnext[w] = nxt;
+        const uint64_t b = bm[w];
+        if (b) nxt = (uint32_t)(w * 64 + __builtin_ctzll(b));
+    }
+    for (int j = 0; j < WPT; j++) {
+        const int w = w0 + j;
+        if (w >= WORDS) break;
+        woff[w] = run;
+        run += (uint32_t)__popcll(bm[w]);
+    }
+    __syncthreads();
+
+    // ---- 3. publish the word count (runs + terminator), look back ----
+    const uint64_t nwords = (uint64_t)runs + 1u;
+    if (threadIdx.x < 64) {
+        uint64_t excl = 0;
+        if (!a.single) {
+            if (k == 0) {
+                if (threadIdx.x == 0) st_relaxed(a.state, kRFlagPre | nwords);
+            } else {
+                if (threadIdx.x == 0) st_relaxed(a.state + k, kRFlagAgg | nwords);
+                int64_t top = (int64_t)k - 1;
+                const uint32_t ln = r_lane();
+                for (;;) {
+                    const int64_t j = top - (int64_t)ln;
+                    const uint64_t s = (j >= 0) ? ld_relaxed(a.state + j) : kRFlagPre;
+                    const uint64_t f = s >> 62;
+                    const uint64_t pre = __ballot(f == 2);
+                    const uint64_t zero = __ballot(f == 0);
+                    const uint64_t need = pre ? (pre & (~pre + 1ull)) * 2ull - 1ull : ~0ull;
+                    if (zero & need) {
+                        __builtin_amdgcn_s_sleep(1);
+                        continue;
+                    }
+                    uint64_t v = ((need >> ln) & 1ull) ? (s & kRValMask) : 0ull;
+#pragma unroll
+                    for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
+                    excl += v;
+                    if (pre) break;
+                    top -= 64;
+                }
+                if (threadIdx.x == 0) st_relaxed(a.state + k, kRFlagPre | (excl + nwords));
+            }
+        }
+        if (threadIdx.x == 0) {
+            *s_base = excl;
+            a.offsets[k] = excl;
+            if (k + 1 == n) a.offsets[n] = excl + nwords;
+            if (a.uniform_out)
+                a.uniform_out[k] = (runs == 1) ? (int16_t)(src ? src[0] : 0) : (int16_t)-1;
+            if (excl + nwords > a.cap) atomicOr(a.status, (uint32_t)VP_STATUS_OVERFLOW);
+        }
+    }
+    __syncthreads();
+    const uint64_t base = *s_base;
+    if (base + nwords > a.cap) return;
+
+    // ---- 4. words ----
+    uint32_t *out = a.words + base;
+    for (int j = 0; j < WPT; j++) {
+        const int w = w0 + j;
+        if (w >= WORDS) break;
+        uint64_t b = bm[w];
+        uint32_t o = woff[w];
+        while (b) {
+            const int bit = __builtin_ctzll(b);
+            b &= b - 1ull;
+            const uint32_t pos = (uint32_t)(w * 64 + bit);
+            const uint32_t end = b ? (uint32_t)(w * 64 + __builtin_ctzll(b)) : wnext[w];
+            out[o++] = ((uint32_t)(src ? src[pos] : 0u) << 24) | (end - pos);
+        }
+    }
+    if (threadIdx.x == 0) out[runs] = 0u;
+}
+
+template <int B, int BS>
+__global__ void __launch_bounds__(BS) k_rle_encode(RleArgs a)
+{
+    constexpr int R = 1 << B, WORDS = R * R * R / 64;
+    __shared__ uint64_t bm[WORDS];
+    __shared__ uint32_t woff[WORDS];
+    __shared__ uint32_t wnext[WORDS];
+    __shared__ uint32_t wsum[BS / 64 + 1];
+    __shared__ uint32_t s_ticket;
+    __shared__ uint64_t s_base;
+    if (a.single) {
+        rle_encode_chunk<B, BS>(a, 0u, 1u, bm, woff, wnext, wsum, &s_base);
+        return;
+    }
+    for (;;) {
+        if (threadIdx.x == 0) s_ticket = atomicAdd(a.ticket, 1u);
+        __syncthreads();
+        const uint32_t k = s_ticket;
+        if (k >= a.n) break;
+        rle_encode_chunk<B, BS>(a, k, a.n, bm, woff, wnext, wsum, &s_base);
+        __syncthreads();
+    }
+}
+
+template <int B, int BS>
+__global__ void __launch_bounds__(BS) k_rle_decode(RleArgs a)
+{
+    constexpr int R = 1 << B, VOX = R * R * R, WORDS = VOX / 64, WPT = (WORDS + BS - 1) / BS;
+    __shared__ uint64_t bm[WORDS];
+    __shared__ uint32_t rank[WORDS];
+    __shared__ uint32_t wsum[BS / 64 + 1];
+    __shared__ uint32_t s_bad, s_nz;
+    for (uint32_t k = blockIdx.x; k < a.n; k += gridDim.x) {
+        const uint32_t chunk = a.ids[k];
+        const uint64_t o0 = a.offsets[k], o1 = a.offsets[k + 1];
+        if (chunk >= a.nchunks || o0 > o1 || o1 > a.cap) {
+            if (threadIdx.x == 0) atomicOr(a.status, (uint32_t)VP_STATUS_BAD_ID);
+            continue;   // uniform across the workgroup
+        }
+        uint8_t *dst = a.voxels + ((size_t)chunk << (3 * B));
+        const uint32_t *wd = a.words + o0;
+        const uint64_t avail = o1 - o0;
+        for (int w = threadIdx.x; w < WORDS; w += BS) bm[w] = 0ull;
+        if (threadIdx.x == 0) s_bad = 0u;
+        __syncthreads();
+        // the runs end at the first 0 word after word 0 (rle.c:100-108: the
+        // first word is always expanded)
+        if (threadIdx.x == 0) s_nz = (uint32_t)(avail < 0xFFFFFFFFull ? avail : 0xFFFFFFFFull);
+        __syncthreads();
+        for (uint64_t i = 1 + threadIdx.x; i < avail; i += BS)
+            if (wd[i] == 0u) atomicMin(&s_nz, (uint32_t)i);
+        __syncthreads();
+        const uint32_t nw = s_nz;
+        // run starts: scan of run lengths, one bit per start
+        uint32_t pos = 0;
+        for (uint32_t t0 = 0; t0 < nw; t0 += BS) {
+            const uint32_t i = t0 + threadIdx.x;
+            const uint32_t len = (i < nw) ? (wd[i] & 0xFFFFFFu) : 0u;
+            if (i < nw && len == 0u) s_bad = 1u;     // zero-length runs are malformed
+            uint32_t tot = 0;
+            const uint32_t st = pos + r_scan_excl<BS>(len, wsum, tot);
+            if (len && st < (uint32_t)VOX)
+                atomicOr(reinterpret_cast<unsigned long long *>(&bm[st >> 6]), 1ull << (st & 63u));
+            pos += tot;
+        }
+        if (threadIdx.x == 0 && pos != (uint32_t)VOX) s_bad = 1u;
+        __syncthreads();
+        // rank of each bitmap word = starts before it
+        uint32_t mine = 0;
+        const int w0 = (int)threadIdx.x * WPT;
+        for (int j = 0; j < WPT; j++)
+            if (w0 + j < WORDS) mine += (uint32_t)__popcll(bm[w0 + j]);
+        uint32_t tot = 0;
+        uint32_t run = r_scan_excl<BS>(mine, wsum, tot);
+        for (int j = 0; j < WPT; j++)
+            if (w0 + j < WORDS) {
+                rank[w0 + j] = run;
+                run += (uint32_t)__popcll(bm[w0 + j]);
+            }
+        __syncthreads();
+        // bytes: run index of byte p = starts at or before p, minus one
+        for (int s = threadIdx.x; s < VOX / 16; s += BS) {
+            const int w = s >> 2;
+            const uint64_t b = bm[w];
+            const int sh = (s & 3) * 16;
+            const uint64_t below = sh ? (b & ((1ull << sh) - 1ull)) : 0ull;
+            int ri = (int)(rank[w] + (uint32_t)__popcll(below)) - 1;
+            const uint32_t bits16 = (uint32_t)(b >> sh) & 0xFFFFu;
+            uint32_t out[4] = {0, 0, 0, 0};
+            const uint32_t p0 = (uint32_t)s * 16;
+            for (int j = 0; j < 16; j++) {
+                if ((bits16 >> j) & 1u) ri++;
+                uint32_t v = 0;
+                if (ri >= 0 && p0 + j < pos) v = wd[ri] >> 24;
+                out[j >> 2] |= v << ((j & 3) * 8);
+            }
+            *reinterpret_cast<uint4 *>(dst + p0) = make_uint4(out[0], out[1], out[2], out[3]);
+        }
+        if (threadIdx.x == 0 && s_bad) atomicOr(a.status, (uint32_t)VP_STATUS_BAD_RLE);
+        __syncthreads();
+    }
+}
+
+__global__ void k_rle_reset(uint32_t *ticket, uint64_t *state, uint32_t n, uint64_t *offsets)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) {
+        ticket[0] = 0u;
+        offsets[0] = 0ull;
+    }
+    for (uint32_t j = i; j < n; j += gridDim.x * blockDim.x) state[j] = 0ull;
+}
+
+template <int B>
+static hipError_t rle_launch(int dir, const RleArgs &a, int grid, hipStream_t st)
+{
+    if (dir == VP_RLE_ENCODE)
+        hipLaunchKernelGGL((k_rle_encode<B, 512>), dim3(grid), dim3(512), 0, st, a);
+    else
+        hipLaunchKernelGGL((k_rle_decode<B, 512>), dim3(grid), dim3(512), 0, st, a);
+    return hipGetLastError();
+}
+
+}  // namespace vp
+
+using namespace vp;
+
+extern "C" int vp_gpu_rle(vp_ctx *ctx, int dir, const vp_world_desc *world,
+                          uint8_t *d_voxels, const uint32_t *d_chunk_ids, uint32_t n,
+                          uint32_t *d_words, uint64_t capacity_words,
+                          uint64_t *d_word_offsets, int16_t *d_uniform_out, void *stream)
+{
+    if (!ctx || !world || !d_voxels || !d_chunk_ids || !d_words || !d_word_offsets)
+        return VP_ERR_ARG;
+    if (dir != VP_RLE_ENCODE && dir != VP_RLE_DECODE) return VP_ERR_ARG;
+    const uint32_t root_bitw = world->root_bitw;
+    if (root_bitw < 2 || root_bitw > 6) return VP_ERR_UNSUPPORTED;
+    if ((int)world->max_bitw[0] + world->max_bitw[1] + world->max_bitw[2] > 32) return VP_ERR_ARG;
+    if (n == 0) return VP_OK;
+    hipStream_t st = as_stream(stream);
+    RleArgs a = {};
+    a.voxels = d_voxels;
+    a.ids = d_chunk_ids;
+    a.n = n;
+    a.words = d_words;
+    a.cap = capacity_words;
+    a.offsets = d_word_offsets;
+    a.uniform_out = d_uniform_out;
+    a.status = ctx->d_status;
+    a.single = (n == 1) ? 1 : 0;
+    a.nchunks = (uint32_t)(1ull << (world->max_bitw[0] + world->max_bitw[1] + world->max_bitw[2]));
+    int grid = 1;
+    if (dir == VP_RLE_ENCODE) {
+        std::lock_guard<std::mutex> g(ctx->mu);
+        if (!a.single) {
+            int rc = ensure_state(ctx, n);
+            if (rc) return rc;
+            a.ticket = ctx->d_ticket;
+            a.state = ctx->d_state;
+            const int resident = ctx->num_cus * 2;
+            grid = (int)(n < (uint32_t)resident ? n : (uint32_t)resident);
+            const int rgrid = (int)((n + 255u) / 256u);
+            hipLaunchKernelGGL(k_rle_reset, dim3(rgrid < 1024 ? rgrid : 1024), dim3(256), 0, st,
+                               a.ticket, a.state, n, a.offsets);
+            VP_HIP_TRY(hipGetLastError());
+        }
+    } else {
+        const int resident = ctx->num_cus * 2;
+        grid = (int)(n < (uint32_t)resident ? n : (uint32_t)resident);
+    }
+    hipError_t e;
+    switch (root_bitw) {
+    case 2: e = rle_launch<2>(dir, a, grid, st); break;
+    case 3: e = rle_launch<3>(dir, a, grid, st); break;
+    case 4: e = rle_launch<4>(dir, a, grid, st); break;
+    case 5: e = rle_launch<5>(dir, a, grid, st); break;
+    default: e = rle_launch<6>(dir, a, grid, st); break;
+    }
+    return e == hipSuccess ? VP_OK : VP_ERR_HIP;
+}

@@ -1,0 +1,355 @@
+// splat.hip -- per-pixel ray/AABB splat resolve and its colour backward, gfx950.
+//
+// Replaces the splat draw of gfx_svl_draw (src/gfx/vsplat.c:104-193): the
+// vertex stage of resources/splat.vert.glsl:34-54 (view translation, clip
+// position, point-sprite size) and the fragment stage of
+// resources/splat.frag.glsl:60-156 (view ray, slab ray/AABB test, depth
+// t0/far, GL_LESS), with GL's point clipping and sprite coverage made
+// explicit:
+//  * a point survives iff its clip-space vertex is inside the clip volume;
+//  * its sprite covers pixel p iff wx - s/2 <= p.x + 0.5 < wx + s/2 (same in y),
+//    s = the point size clamped to [1, 2048];
+//  * the nearest hit wins, ties to the lower splat index (GL submission
+//    order): one 64-bit atomicMin per fragment on (depth bits << 32 | index).
+// All float arithmetic is written one IEEE operation at a time, contraction
+// off, correctly rounded division and square root, GLSL min/max as selects,
+// so the results are those of the stated operation order.
+#pragma clang fp contract(off)
+
+#include "vp_internal.h"
+
+namespace vp {
+
+struct SplatCam {
+    float T[16], PR[16], IPR[16];
+    uint32_t W, H;
+    float far_plane;
+};
+
+__device__ __forceinline__ void mv4(const float *m, const float v[4], float o[4])
+{
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+        float s = m[0 * 4 + r] * v[0];
+        s = s + m[1 * 4 + r] * v[1];
+        s = s + m[2 * 4 + r] * v[2];
+        s = s + m[3 * 4 + r] * v[3];
+        o[r] = s;
+    }
+}
+
+__device__ __forceinline__ float gmin(float x, float y) { return (y < x) ? y : x; }
+__device__ __forceinline__ float gmax(float x, float y) { return (x < y) ? y : x; }
+__device__ __forceinline__ float gsign(float x)
+{
+    return x > 0.0f ? 1.0f : (x < 0.0f ? -1.0f : 0.0f);
+}
+
+struct VS {
+    float vx, vy, vz, half;
+    float lox, hix, loy, hiy;
+    int x0, x1, y0, y1;
+};
+
+__device__ __forceinline__ bool vertex_stage(const int16_t *rec, float lodf, const SplatCam &c,
+                                             VS &o)
+{
+    const float half = 0.5f * lodf;
+    const float a[4] = {(float)rec[0] + half, (float)rec[1] + half, (float)rec[2] + half, 1.0f};
+    float v[4], clip[4];
+    mv4(c.T, a, v);
+    mv4(c.PR, v, clip);
+    const float W = (float)c.W, H = (float)c.H, cw = clip[3];
+    if (!(clip[0] >= -cw && clip[0] <= cw && clip[1] >= -cw && clip[1] <= cw &&
+          clip[2] >= -cw && clip[2] <= cw))
+        return false;
+    const float ratio = W / H;
+    const float rx = fabsf((clip[0] * ratio) / cw);
+    const float ry = fabsf(clip[1] / cw);
+    float reduce = gmax(rx, ry);
+    reduce = reduce + 0.03f;
+    float size = (H * 1.1f) / clip[2];
+    size = size * gmax(reduce, 1.0f);
+    float ps = size * lodf;
+    ps = (ps > 1.0f) ? ps : 1.0f;
+    ps = (ps < 2048.0f) ? ps : 2048.0f;
+    const float ndx = clip[0] / cw, ndy = clip[1] / cw;
+    const float wx = (ndx + 1.0f) * (0.5f * W);
+    const float wy = (ndy + 1.0f) * (0.5f * H);
+    const float hs = 0.5f * ps;
+    o.lox = wx - hs; o.hix = wx + hs;
+    o.loy = wy - hs; o.hiy = wy + hs;
+    const int x0 = (int)floorf(o.lox) - 1, x1 = (int)ceilf(o.hix) + 1;
+    const int y0 = (int)floorf(o.loy) - 1, y1 = (int)ceilf(o.hiy) + 1;
+    o.x0 = x0 < 0 ? 0 : x0;
+    o.y0 = y0 < 0 ? 0 : y0;
+    o.x1 = x1 > (int)c.W - 1 ? (int)c.W - 1 : x1;
+    o.y1 = y1 > (int)c.H - 1 ? (int)c.H - 1 : y1;
+    o.vx = v[0]; o.vy = v[1]; o.vz = v[2];
+    o.half = half;
+    return true;
+}
+
+__device__ __forceinline__ void pixel_dir(const SplatCam &c, int px, int py, float d[3])
+{
+    const float fx = ((float)px + 0.5f) / (float)c.W;
+    const float fy = ((float)py + 0.5f) / (float)c.H;
+    const float cl[4] = {fx * 2.0f - 1.0f, fy * 2.0f - 1.0f, -1.0f, 1.0f};
+    float r[4];
+    mv4(c.IPR, cl, r);
+    const float x = r[0] / r[3], y = r[1] / r[3], z = r[2] / r[3];
+    float dd = x * x;
+    dd = dd + y * y;
+    dd = dd + z * z;
+    const float len = sqrtf(dd);
+    d[0] = x / len; d[1] = y / len; d[2] = z / len;
+}
+
+__device__ __forceinline__ void aabb(float vx, float vy, float vz, float half, float ix, float iy,
+                                     float iz, float &t0, float &t1)
+{
+    const float tbx = (vx - half) * ix, ttx = (vx + half) * ix;
+    const float tby = (vy - half) * iy, tty = (vy + half) * iy;
+    const float tbz = (vz - half) * iz, ttz = (vz + half) * iz;
+    const float mnx = gmin(ttx, tbx), mny = gmin(tty, tby), mnz = gmin(ttz, tbz);
+    const float mxx = gmax(ttx, tbx), mxy = gmax(tty, tby), mxz = gmax(ttz, tbz);
+    t0 = gmax(gmax(mnx, mny), gmax(mnx, mnz));
+    t1 = gmin(gmin(mxx, mxy), gmin(mxx, mxz));
+}
+
+__global__ void k_rays(SplatCam c, float4 *rays)
+{
+    const uint32_t px = blockIdx.x * blockDim.x + threadIdx.x, py = blockIdx.y;
+    if (px >= c.W) return;
+    float d[3];
+    pixel_dir(c, (int)px, (int)py, d);
+    rays[(size_t)py * c.W + px] = make_float4(1.0f / d[0], 1.0f / d[1], 1.0f / d[2], 0.0f);
+}
+
+__global__ void k_clear(uint64_t *frame, uint64_t np)
+{
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < np;
+         i += (uint64_t)gridDim.x * blockDim.x)
+        frame[i] = ~0ull;
+}
+
+__device__ __forceinline__ void fragment(const SplatCam &c, const float4 *rays, uint64_t *frame,
+                                         float vx, float vy, float vz, float half, uint32_t id,
+                                         int px, int py)
+{
+    const size_t p = (size_t)py * c.W + (uint32_t)px;
+    const float4 inv = rays[p];
+    float t0, t1;
+    aabb(vx, vy, vz, half, inv.x, inv.y, inv.z, t0, t1);
+    if (!(t0 <= t1)) return;
+    float d = t0 / c.far_plane;
+    if (!(d < 1.0f)) return;
+    if (d <= 0.0f) d = 0.0f;
+    const uint64_t key = ((uint64_t)__float_as_uint(d) << 32) | id;
+    if (key < frame[p]) atomicMin(reinterpret_cast<unsigned long long *>(frame + p), key);
+}
+
+constexpr int kSmallArea = 16;
+
+// One thread per splat; sprites larger than kSmallArea pixels are shared by
+// the whole wave, one at a time, so a few near splats do not stall a wave.
+__global__ void __launch_bounds__(256) k_raster(const int16_t *splats, const uint8_t *lod,
+                                                uint32_t lod_log2, uint64_t n, SplatCam c,
+                                                const float4 *rays, uint64_t *frame)
+{
+    const uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    VS vs;
+    bool ok = false;
+    if (i < n) {
+        const float lodf = (float)(1u << (lod ? lod[i] : lod_log2));
+        ok = vertex_stage(splats + 4 * i, lodf, c, vs);
+    }
+    int area = 0;
+    if (ok && vs.x0 <= vs.x1 && vs.y0 <= vs.y1) area = (vs.x1 - vs.x0 + 1) * (vs.y1 - vs.y0 + 1);
+    const uint32_t id = (uint32_t)i;
+    if (area > 0 && area <= kSmallArea) {
+        for (int py = vs.y0; py <= vs.y1; py++) {
+            const float cy = (float)py + 0.5f;
+            if (!(vs.loy <= cy && cy < vs.hiy)) continue;
+            for (int px = vs.x0; px <= vs.x1; px++) {
+                const float cx = (float)px + 0.5f;
+                if (!(vs.lox <= cx && cx < vs.hix)) continue;
+                fragment(c, rays, frame, vs.vx, vs.vy, vs.vz, vs.half, id, px, py);
+            }
+        }
+    }
+    uint64_t big = __ballot(area > kSmallArea);
+    const int lane = (int)(threadIdx.x & 63u);
+    while (big) {
+        const int src = __builtin_ctzll(big);
+        big &= big - 1ull;
+        const float vx = __shfl(vs.vx, src, 64), vy = __shfl(vs.vy, src, 64);
+        const float vz = __shfl(vs.vz, src, 64), half = __shfl(vs.half, src, 64);
+        const float lox = __shfl(vs.lox, src, 64), hix = __shfl(vs.hix, src, 64);
+        const float loy = __shfl(vs.loy, src, 64), hiy = __shfl(vs.hiy, src, 64);
+        const int x0 = __shfl(vs.x0, src, 64), x1 = __shfl(vs.x1, src, 64);
+        const int y0 = __shfl(vs.y0, src, 64);
+        const int a = __shfl(area, src, 64);
+        const uint32_t sid = __shfl(id, src, 64);
+        const int w = x1 - x0 + 1;
+        for (int k = lane; k < a; k += 64) {
+            const int px = x0 + k % w, py = y0 + k / w;
+            const float cx = (float)px + 0.5f, cy = (float)py + 0.5f;
+            if (!(lox <= cx && cx < hix && loy <= cy && cy < hiy)) continue;
+            fragment(c, rays, frame, vx, vy, vz, half, sid, px, py);
+        }
+    }
+}
+
+__global__ void k_resolve(const int16_t *splats, const uint8_t *lod, uint32_t lod_log2,
+                          uint64_t n, SplatCam c, const float *voxel_rgb, const uint64_t *frame,
+                          const float4 *rays, float *depth, uint32_t *idbuf, float *rgb,
+                          float *weight)
+{
+    const uint32_t px = blockIdx.x * blockDim.x + threadIdx.x, py = blockIdx.y;
+    if (px >= c.W) return;
+    const size_t p = (size_t)py * c.W + px;
+    const uint64_t key = frame[p];
+    if (key == ~0ull) {
+        depth[p] = 1.0f;
+        idbuf[p] = 0xFFFFFFFFu;
+        if (rgb) { rgb[3 * p] = 0.5f; rgb[3 * p + 1] = 0.5f; rgb[3 * p + 2] = 0.7f; }
+        if (weight) weight[p] = 0.0f;
+        return;
+    }
+    const uint32_t i = (uint32_t)key;
+    const float d = __uint_as_float((uint32_t)(key >> 32));
+    depth[p] = d;
+    idbuf[p] = i;
+    if (!rgb && !weight) return;
+    // shading, splat.vert.glsl:24-32 and splat.frag.glsl:93-154
+    const float lodf = (float)(1u << (lod ? lod[i] : lod_log2));
+    const float half = 0.5f * lodf;
+    const int16_t *rec = splats + 4 * (size_t)i;
+    const float a[4] = {(float)rec[0] + half, (float)rec[1] + half, (float)rec[2] + half, 1.0f};
+    float v[4];
+    mv4(c.T, a, v);
+    float dir[3];
+    pixel_dir(c, (int)px, (int)py, dir);
+    const float4 inv = rays[p];
+    float t0, t1;
+    aabb(v[0], v[1], v[2], half, inv.x, inv.y, inv.z, t0, t1);
+    float hit[3], ha[3];
+    for (int k = 0; k < 3; k++) {
+        hit[k] = v[k] - dir[k] * t0;
+        ha[k] = fabsf(hit[k]);
+    }
+    const float md = gmax(gmax(ha[0], ha[1]), ha[2]);
+    float nrm[3];
+    for (int k = 0; k < 3; k++) nrm[k] = (ha[k] == md ? 1.0f : 0.0f) * gsign(hit[k]);
+    const float lx = -0.70710677f, ly = 0.70710677f, lz = 0.0f;
+    float nd = (-nrm[0]) * lx;
+    nd = nd + (-nrm[1]) * ly;
+    nd = nd + (-nrm[2]) * lz;
+    const int attr = rec[3];
+    const float sh = (float)((attr >> 6) & 1);
+    const float s1 = gmax(0.7f, 1.0f - sh);
+    const float s2 = gmax(gmax(0.7f, sh), gsign(nd));
+    float base[3];
+    if (voxel_rgb) {
+        for (int k = 0; k < 3; k++) base[k] = voxel_rgb[3 * (size_t)i + k];
+    } else {
+        base[0] = (float)((attr >> 4) & 3) / 3.0f;
+        base[1] = (float)((attr >> 2) & 3) / 3.0f;
+        base[2] = (float)(attr & 3) / 3.0f;
+    }
+    const float fs = 256.0f / c.far_plane;
+    const float fog = sqrtf(gmax(d - fs, 0.0001f));
+    const float keep = 1.0f - fog;
+    const float fogc[3] = {0.5f, 0.5f, 0.7f};
+    if (rgb)
+        for (int k = 0; k < 3; k++) {
+            const float vcol = base[k] * s1;
+            const float col = vcol * s2;
+            rgb[3 * p + k] = col * keep + fogc[k] * fog;
+        }
+    if (weight) weight[p] = (s1 * s2) * keep;
+}
+
+__global__ void k_backward(uint64_t np, uint64_t n, const uint32_t *idbuf, const float *weight,
+                           const float *dL, float *grad)
+{
+    for (uint64_t p = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; p < np;
+         p += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t i = idbuf[p];
+        if (i == 0xFFFFFFFFu || i >= n) continue;
+        const float w = weight[p];
+        if (w == 0.0f) continue;
+        for (int k = 0; k < 3; k++) atomicAdd(grad + 3 * (size_t)i + k, w * dL[3 * p + k]);
+    }
+}
+
+static SplatCam to_cam(const vp_camera *cam)
+{
+    SplatCam c;
+    for (int i = 0; i < 16; i++) {
+        c.T[i] = cam->view_translation[i];
+        c.PR[i] = cam->proj_view_rot[i];
+        c.IPR[i] = cam->inv_proj_view_rot[i];
+    }
+    c.W = cam->width;
+    c.H = cam->height;
+    c.far_plane = cam->far_plane;
+    return c;
+}
+
+}  // namespace vp
+
+using namespace vp;
+
+extern "C" int vp_gpu_splat(vp_ctx *ctx, const int16_t *d_splats, const uint8_t *d_lod,
+                            uint32_t lod_log2, uint64_t n, const vp_camera *cam,
+                            const float *d_voxel_rgb, const vp_frame *out, void *stream)
+{
+    if (!ctx || !cam || !out || !out->d_depth || !out->d_id) return VP_ERR_ARG;
+    if (n && !d_splats) return VP_ERR_ARG;
+    if (cam->width == 0 || cam->height == 0 || cam->width > 16384 || cam->height > 16384)
+        return VP_ERR_ARG;
+    if (lod_log2 > 8) return VP_ERR_ARG;
+    if (n > 0xFFFFFFFFull) return VP_ERR_UNSUPPORTED;   // ids are 32-bit
+    const uint64_t np = (uint64_t)cam->width * cam->height;
+    hipStream_t st = as_stream(stream);
+    uint64_t *frame;
+    float4 *rays;
+    {
+        std::lock_guard<std::mutex> g(ctx->mu);
+        int rc = ensure_frame(ctx, np);
+        if (rc) return rc;
+        frame = ctx->d_frame;
+        rays = ctx->d_rays;
+    }
+    const SplatCam c = to_cam(cam);
+    const dim3 pg((cam->width + 255) / 256, cam->height);
+    hipLaunchKernelGGL(k_rays, pg, dim3(256), 0, st, c, rays);
+    hipLaunchKernelGGL(k_clear, dim3(1024), dim3(256), 0, st, frame, np);
+    if (n) {
+        const uint64_t blocks = (n + 255) / 256;
+        hipLaunchKernelGGL(k_raster, dim3((uint32_t)blocks), dim3(256), 0, st, d_splats, d_lod,
+                           lod_log2, n, c, rays, frame);
+    }
+    hipLaunchKernelGGL(k_resolve, pg, dim3(256), 0, st, d_splats, d_lod, lod_log2, n, c,
+                       d_voxel_rgb, frame, rays, out->d_depth, out->d_id, out->d_rgb,
+                       out->d_weight);
+    VP_HIP_TRY(hipGetLastError());
+    return VP_OK;
+}
+
+extern "C" int vp_gpu_splat_backward(vp_ctx *ctx, uint64_t n, const vp_camera *cam,
+                                     const uint32_t *d_id, const float *d_weight,
+                                     const float *d_dL_drgb, float *d_grad, void *stream)
+{
+    if (!ctx || !cam || !d_id || !d_weight || !d_dL_drgb || !d_grad) return VP_ERR_ARG;
+    const uint64_t np = (uint64_t)cam->width * cam->height;
+    if (np == 0) return VP_OK;
+    uint64_t blocks = (np + 255) / 256;
+    if (blocks > 4096) blocks = 4096;
+    hipLaunchKernelGGL(k_backward, dim3((uint32_t)blocks), dim3(256), 0, as_stream(stream), np, n,
+                       d_id, d_weight, d_dL_drgb, d_grad);
+    VP_HIP_TRY(hipGetLastError());
+    return VP_OK;
+}

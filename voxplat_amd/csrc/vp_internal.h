@@ -1,0 +1,53 @@
+// vp_internal.h -- shared definitions of the HIP library (not part of the ABI).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <mutex>
+
+#include "../../include/vp_gpu.h"
+
+#define VP_HIP_TRY(expr)                                   \
+    do {                                                   \
+        hipError_t e_ = (expr);                            \
+        if (e_ != hipSuccess) return VP_ERR_HIP;           \
+    } while (0)
+
+struct vp_ctx {
+    int device = 0;
+    int num_cus = 0;
+    int xcds = 8;
+    std::mutex mu;
+    // device scratch: [0] status word, [1] ticket counter, then look-back
+    // state words (u64).  Grown on demand, never shrunk.
+    uint32_t *d_status = nullptr;      // 1 word
+    uint32_t *d_ticket = nullptr;      // 4 words
+    uint64_t *d_state = nullptr;       // state_cap words
+    uint64_t  state_cap = 0;
+    uint64_t *d_frame = nullptr;       // packed depth|id per pixel for splat
+    uint64_t  frame_cap = 0;
+    float4   *d_rays = nullptr;        // per-pixel 1/ray direction for splat
+    uint64_t  rays_cap = 0;
+    // tuned launch shape of the persistent remesh kernel
+    int cull_block = 512;              // threads per workgroup
+    int cull_wg_per_cu = 2;            // resident workgroups per CU
+};
+
+namespace vp {
+
+int ensure_state(vp_ctx *ctx, uint64_t words);
+int ensure_frame(vp_ctx *ctx, uint64_t pixels);
+
+inline hipStream_t as_stream(void *s) { return reinterpret_cast<hipStream_t>(s); }
+
+// relaxed device-scope atomics on 64-bit words (coherent across XCDs)
+__device__ __forceinline__ uint64_t ld_relaxed(const uint64_t *p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_relaxed(uint64_t *p, uint64_t v)
+{
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+}  // namespace vp
