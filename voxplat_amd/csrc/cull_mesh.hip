@@ -70,24 +70,41 @@ struct Geo {
     }
 };
 
-// LDS layout in 64-bit words.  The per-row offsets (u32) reuse the occupancy
-// rows once visibility is built, unless v2 face flags still need them.
+// LDS layout in 64-bit words.  Once visibility is built the occupancy rows
+// are dead (unless v2 face flags still read them) and their space holds the
+// emission scratch: slice offsets, the staged slice pair, the row offsets of
+// the rows being expanded and the list of their cells.
 template <int B, bool V2>
 struct Lay {
     using G = Geo<B>;
-    static constexpr int R = G::R;
+    static constexpr int R = G::R, N0 = G::N0;
     static constexpr int S = 0;
     static constexpr int HX = S + R * R;
     static constexpr int HY = HX + R;
     static constexpr int HZ = HY + R;
     static constexpr int V0 = HZ + R;
-    static constexpr int XB = V0 + G::N0 * G::N0;
+    static constexpr int XB = V0 + N0 * N0;
     static constexpr int P1 = XB + R;               // P1..P4 contiguous, row order
     static constexpr int PEND = P1 + (G::ROWS - G::RB1);
-    static constexpr int ROWOFF_WORDS = (G::ROWS + 1) / 2;
-    static constexpr bool ALIAS = !V2 && (ROWOFF_WORDS <= R * R);
-    static constexpr int ROWOFF = ALIAS ? S : PEND;
-    static constexpr int TOTAL = ALIAS ? PEND : PEND + ROWOFF_WORDS;
+    // z-slices of every level, level-major: slice base of level l
+    static constexpr int SLB0 = 0, SLB1 = G::n(0), SLB2 = SLB1 + G::n(1), SLB3 = SLB2 + G::n(2),
+                         SLB4 = SLB3 + G::n(3), NSLICES = SLB4 + G::n(4);
+    // staged slice pair: two z-slices, +x column and +y row bytes, 3 shadow rows
+    static constexpr int SW = 2 * R + 8;              // shadow entries per staged row
+    static constexpr int ST_BYTES = 2 * R * R + 4 * R + 3 * SW * 2;
+    static constexpr int maxi(int a, int b) { return a > b ? a : b; }
+    static constexpr int al16(int v) { return (v + 15) & ~15; }
+    static constexpr int PR_ROWS = maxi(2 * N0 + G::n(1), G::n(2) * G::n(2));
+    static constexpr int LIST_N = maxi(2 * N0 * N0 + G::n(1) * G::n(1),
+                                       G::n(2) * G::n(2) * G::n(2));
+    static constexpr int O_STG = 0;
+    static constexpr int O_SOFF = al16(ST_BYTES);
+    static constexpr int O_ROWP = O_SOFF + al16((NSLICES + 1) * 4);
+    static constexpr int O_LIST = O_ROWP + al16(PR_ROWS * 2);
+    static constexpr int SCR_BYTES = O_LIST + al16(LIST_N * 2);
+    static constexpr bool ALIAS = !V2 && SCR_BYTES <= R * R * 8;
+    static constexpr int SCR = ALIAS ? S : PEND;
+    static constexpr int TOTAL = ALIAS ? PEND : PEND + SCR_BYTES / 8;
 };
 
 __device__ __forceinline__ uint64_t compress_pairs(uint64_t r)
@@ -114,10 +131,38 @@ __device__ __forceinline__ uint32_t nz16(uint4 v)
 
 __device__ __forceinline__ uint32_t lane_id() { return threadIdx.x & 63u; }
 
+// wave-uniform values read from LDS: keep them in scalar registers
+__device__ __forceinline__ uint32_t uni(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ uint64_t uni(uint64_t v)
+{
+    return ((uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)(v >> 32)) << 32) |
+           __builtin_amdgcn_readfirstlane((uint32_t)v);
+}
+
 __device__ __forceinline__ uint32_t rank_below(uint64_t m)
 {
     return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
                                      __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+
+// position of the k-th set bit (k counted from 0) of v; v has more than k bits
+__device__ __forceinline__ int select64(uint64_t v, uint32_t k)
+{
+    int pos = 0;
+    uint32_t c = __popc((uint32_t)v);
+    if (k >= c) { k -= c; v >>= 32; pos += 32; }
+    c = __popc((uint32_t)v & 0xFFFFu);
+    if (k >= c) { k -= c; v >>= 16; pos += 16; }
+    c = __popc((uint32_t)v & 0xFFu);
+    if (k >= c) { k -= c; v >>= 8; pos += 8; }
+    c = __popc((uint32_t)v & 0xFu);
+    if (k >= c) { k -= c; v >>= 4; pos += 4; }
+    c = __popc((uint32_t)v & 0x3u);
+    if (k >= c) { k -= c; v >>= 2; pos += 2; }
+    c = (uint32_t)v & 1u;
+    if (k >= c) pos += 1;
+    return pos;
 }
 
 struct ChunkRefs {
@@ -125,6 +170,22 @@ struct ChunkRefs {
     int own_u, nb_u[3];           // uniform value or -1
     uint32_t org[3];              // world origin of the chunk (offset * R)
 };
+
+// Buffer resource over [p, p + bytes): loads share one VGPR offset, take the
+// per-load step from a scalar offset, and read 0 past the end.  0x00020000 is
+// the gfx9 word-3 format of a raw byte buffer.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t byte_rsrc(const void *p, uint32_t bytes)
+{
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), (short)0, (int)bytes,
+                                             0x00020000);
+}
+
+__device__ __forceinline__ uint4 ld16(__amdgpu_buffer_rsrc_t rs, uint32_t voff, uint32_t soff)
+{
+    typedef unsigned int v4u __attribute__((ext_vector_type(4)));
+    const v4u v = __builtin_amdgcn_raw_buffer_load_b128(rs, (int)voff, (int)soff, 0);
+    return make_uint4(v.x, v.y, v.z, v.w);
+}
 
 // Fill `dst` (row words, rowcount rows of R bits) from R-byte rows at src with
 // row stride `stride` (bytes).  Rows are numbered r = 0..rowcount-1 and row r
@@ -134,35 +195,41 @@ __device__ __forceinline__ void load_rows(uint64_t *dst, const uint8_t *src, int
                                           int stride)
 {
     constexpr int R = 1 << B;
-    if constexpr (R >= 16) {
+    if constexpr (R == 64) {
+        uint16_t *d16 = reinterpret_cast<uint16_t *>(dst);
+        const int total = rowcount * 4;
+        if (stride == R) {
+            // contiguous rows: byte offset = 16 * index, one buffer for all
+            constexpr int DEPTH = 8;
+            const __amdgpu_buffer_rsrc_t rs = byte_rsrc(src, (uint32_t)total * 16u);
+            int i = threadIdx.x;
+#pragma unroll 1
+            for (; i + (DEPTH - 1) * BS < total; i += DEPTH * BS) {
+                uint4 v[DEPTH];
+#pragma unroll
+                for (int j = 0; j < DEPTH; j++) v[j] = ld16(rs, (uint32_t)i * 16u, (uint32_t)(j * BS * 16));
+#pragma unroll
+                for (int j = 0; j < DEPTH; j++) d16[i + j * BS] = (uint16_t)nz16(v[j]);
+            }
+#pragma unroll 1
+            for (; i < total; i += BS) d16[i] = (uint16_t)nz16(ld16(rs, (uint32_t)i * 16u, 0u));
+        } else {
+            for (int i = threadIdx.x; i < total; i += BS) {
+                const int r = i >> 2, p = i & 3;
+                const uint4 v = *reinterpret_cast<const uint4 *>(src + (size_t)r * stride + p * 16);
+                d16[i] = (uint16_t)nz16(v);
+            }
+        }
+    } else if constexpr (R >= 16) {
         constexpr int PARTS = R / 16;
         uint16_t *d16 = reinterpret_cast<uint16_t *>(dst);
         const int total = rowcount * PARTS;
-        int i = threadIdx.x;
-        // keep several 16-byte loads in flight per lane
-        for (; i + 3 * BS < total; i += 4 * BS) {
-            uint4 v[4];
-#pragma unroll
-            for (int j = 0; j < 4; j++) {
-                int idx = i + j * BS, r = idx / PARTS, p = idx % PARTS;
-                v[j] = *reinterpret_cast<const uint4 *>(src + (size_t)r * stride + p * 16);
-            }
-#pragma unroll
-            for (int j = 0; j < 4; j++) {
-                int idx = i + j * BS, r = idx / PARTS, p = idx % PARTS;
-                d16[r * 4 + p] = (uint16_t)nz16(v[j]);
-                if (PARTS < 4 && p == PARTS - 1) {
-                    for (int q = PARTS; q < 4; q++) d16[r * 4 + q] = 0;
-                }
-            }
-        }
-        for (; i < total; i += BS) {
-            int r = i / PARTS, p = i % PARTS;
-            uint4 v = *reinterpret_cast<const uint4 *>(src + (size_t)r * stride + p * 16);
+        for (int i = threadIdx.x; i < total; i += BS) {
+            const int r = i / PARTS, p = i % PARTS;
+            const uint4 v = *reinterpret_cast<const uint4 *>(src + (size_t)r * stride + p * 16);
             d16[r * 4 + p] = (uint16_t)nz16(v);
-            if (PARTS < 4 && p == PARTS - 1) {
+            if (p == PARTS - 1)
                 for (int q = PARTS; q < 4; q++) d16[r * 4 + q] = 0;
-            }
         }
     } else {
         for (int r = threadIdx.x; r < rowcount; r += BS) {
@@ -299,8 +366,157 @@ __device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t *wsum, 
         if (lane == (uint32_t)NW - 1) wsum[NW] = t;       // block total
     }
     __syncthreads();
-    total = wsum[NW];
+    total = uni(wsum[NW]);
     return wsum[w] + x - v;
+}
+
+// ---- emission helpers --------------------------------------------------------
+// Registers carrying one slice pair from global memory to the LDS staging.
+template <int B, int BS>
+struct PairRegs {
+    static constexpr int R = 1 << B;
+    static constexpr int NSL = (2 * R * R + 15) / 16;   // 16-byte pieces of two slices
+    static constexpr int NXY = 4 * R;                   // +x column and +y row bytes
+    static constexpr int NSH = 3 * (2 * R + 8);         // shadow entries
+    static constexpr int KSL = (NSL + BS - 1) / BS;
+    static constexpr int KXY = (NXY + BS - 1) / BS;
+    static constexpr int KSH = (NSH + BS - 1) / BS;
+    uint4 sl[KSL];
+    uint32_t xy[KXY];
+    uint32_t sh[KSH];
+};
+
+__device__ __forceinline__ uint4 splat4(uint32_t v)
+{
+    const uint32_t w = (v & 0xFFu) * 0x01010101u;
+    return make_uint4(w, w, w, w);
+}
+
+// Issue the loads of pair (z0, z0+1); slice z = R is the +z neighbour's z=0 plane.
+template <int B, int BS>
+__device__ __forceinline__ void pair_load(const CullArgs &a, const ChunkRefs &c, int z0,
+                                          PairRegs<B, BS> &pr)
+{
+    constexpr int R = 1 << B, SL = R * R;
+    using P = PairRegs<B, BS>;
+#pragma unroll
+    for (int j = 0; j < P::KSL; j++) {
+        const int i = (int)threadIdx.x + j * BS;
+        uint4 v = make_uint4(0, 0, 0, 0);
+        if (i < P::NSL) {
+            const int byte = i * 16, sidx = byte / SL, off = byte % SL, z = z0 + sidx;
+            if (SL >= 16) {
+                if (z < R) {
+                    v = c.own_u >= 0 ? splat4((uint32_t)c.own_u)
+                                     : *reinterpret_cast<const uint4 *>(c.own + (size_t)z * SL + off);
+                } else if (z == R) {
+                    v = c.nb_u[2] >= 0 ? splat4((uint32_t)c.nb_u[2])
+                                       : *reinterpret_cast<const uint4 *>(c.nb[2] + off);
+                }
+            } else {   // slices smaller than 16 bytes (R = 2): byte by byte
+                uint32_t wv[4] = {0, 0, 0, 0};
+                for (int q = 0; q < 16; q++) {
+                    const int bb = byte + q, zz = z0 + bb / SL, oo = bb % SL;
+                    uint32_t x = 0;
+                    if (zz < R) x = c.own_u >= 0 ? (uint32_t)c.own_u : c.own[(size_t)zz * SL + oo];
+                    else if (zz == R) x = c.nb_u[2] >= 0 ? (uint32_t)c.nb_u[2] : c.nb[2][oo];
+                    wv[q >> 2] |= (x & 0xFFu) << (8 * (q & 3));
+                }
+                v = make_uint4(wv[0], wv[1], wv[2], wv[3]);
+            }
+        }
+        pr.sl[j] = v;
+    }
+#pragma unroll
+    for (int j = 0; j < P::KXY; j++) {
+        const int i = (int)threadIdx.x + j * BS;
+        uint32_t v = 0;
+        if (i < P::NXY) {
+            const int kind = i / (2 * R), sidx = (i % (2 * R)) / R, e = i % R, z = z0 + sidx;
+            if (z < R) {
+                if (kind == 0)   // voxel (0, e, z) of the +x neighbour
+                    v = c.nb_u[0] >= 0 ? (uint32_t)c.nb_u[0] : c.nb[0][(size_t)(z * R + e) * R];
+                else             // voxel (e, 0, z) of the +y neighbour
+                    v = c.nb_u[1] >= 0 ? (uint32_t)c.nb_u[1] : c.nb[1][(size_t)(z * R) * R + e];
+            }
+        }
+        pr.xy[j] = v;
+    }
+#pragma unroll
+    for (int j = 0; j < P::KSH; j++) {
+        const int i = (int)threadIdx.x + j * BS;
+        uint32_t v = 0;
+        if (i < P::NSH && a.map) {
+            const int SWn = 2 * R + 8, row = i / SWn, e = i % SWn;
+            v = a.map[(size_t)a.sx * (c.org[2] + z0 + row) + (c.org[0] + c.org[1]) + e];
+        }
+        pr.sh[j] = v;
+    }
+}
+
+template <int B, int BS>
+__device__ __forceinline__ void pair_store(const PairRegs<B, BS> &pr, uint8_t *stg)
+{
+    constexpr int R = 1 << B;
+    using P = PairRegs<B, BS>;
+    constexpr int XC = 2 * R * R, SH = XC + 4 * R;
+#pragma unroll
+    for (int j = 0; j < P::KSL; j++) {
+        const int i = (int)threadIdx.x + j * BS;
+        if (i < P::NSL) *reinterpret_cast<uint4 *>(stg + i * 16) = pr.sl[j];
+    }
+#pragma unroll
+    for (int j = 0; j < P::KXY; j++) {
+        const int i = (int)threadIdx.x + j * BS;
+        if (i < P::NXY) stg[XC + i] = (uint8_t)pr.xy[j];
+    }
+#pragma unroll
+    for (int j = 0; j < P::KSH; j++) {
+        const int i = (int)threadIdx.x + j * BS;
+        if (i < P::NSH) reinterpret_cast<uint16_t *>(stg + SH)[i] = (uint16_t)pr.sh[j];
+    }
+}
+
+// Colour of level-0 cell (x, y, z) of the staged pair z0, z0+1.
+template <int B>
+__device__ __forceinline__ uint32_t staged_colour(const uint8_t *stg, int x, int y, int z, int z0)
+{
+    constexpr int R = 1 << B;
+    const int sidx = z - z0;
+    if (x == R) return stg[2 * R * R + sidx * R + y];
+    if (y == R) return stg[2 * R * R + 2 * R + sidx * R + x];
+    return stg[sidx * R * R + y * R + x];
+}
+
+// shadow_sample on a staged row: entry e is map[sx*(oz+z0+row) + ox+oy + e]
+template <int B>
+__device__ __forceinline__ uint32_t staged_shadow(const uint8_t *stg, int row, int e, uint32_t wy)
+{
+    constexpr int R = 1 << B;
+    const uint16_t *sh = reinterpret_cast<const uint16_t *>(stg + 2 * R * R + 4 * R);
+    const uint16_t *rw = sh + row * (2 * R + 8);
+    return !(rw[e] <= wy && rw[e + 1] <= wy);
+}
+
+// Exclusive scan of the counts of rows [0, nrows) into rowp (u16, relative
+// to row 0); each thread sums a contiguous block of rows.  Returns the total.
+template <int BS, typename F>
+__device__ __forceinline__ uint32_t scan_rows(int nrows, F count, uint16_t *rowp, uint32_t *wsum)
+{
+    const int K = (nrows + BS - 1) / BS;
+    const int r0 = (int)threadIdx.x * K;
+    uint32_t mine = 0;
+    for (int j = 0; j < K; j++)
+        if (r0 + j < nrows) mine += count(r0 + j);
+    uint32_t tot = 0;
+    uint32_t run = block_excl_scan<BS>(mine, wsum, tot);
+    for (int j = 0; j < K; j++)
+        if (r0 + j < nrows) {
+            rowp[r0 + j] = (uint16_t)run;
+            run += count(r0 + j);
+        }
+    __syncthreads();
+    return tot;
 }
 
 // Work for one chunk; all threads of the workgroup call it.
@@ -314,7 +530,7 @@ __device__ void cull_chunk(const CullArgs &a, uint32_t k, uint32_t n, uint64_t *
     constexpr int NW = BS / 64;
     constexpr uint64_t RM = G::ROWMASK;
 
-    uint32_t chunk = a.ids[k];
+    uint32_t chunk = uni(a.ids[k]);
     const bool bad_id = chunk >= a.nchunks;
     if (bad_id) {
         if (threadIdx.x == 0) atomicOr(a.status, (uint32_t)VP_STATUS_BAD_ID);
@@ -373,6 +589,7 @@ __device__ void cull_chunk(const CullArgs &a, uint32_t k, uint32_t n, uint64_t *
         if (c.nb_u[0] >= 0) {
             for (int z = threadIdx.x; z < R; z += BS) lds[L::HX + z] = c.nb_u[0] ? RM : 0ull;
         } else {
+#pragma unroll 1
             for (int base = 0; base < R * R; base += BS) {
                 const int i = base + (int)threadIdx.x;
                 const bool act = i < R * R;
@@ -385,6 +602,7 @@ __device__ void cull_chunk(const CullArgs &a, uint32_t k, uint32_t n, uint64_t *
         __syncthreads();
 
         // ---- 2. level-0 visibility (mesher.c:421-448 as row algebra) ----
+#pragma unroll 1
         for (int base = 0; base < R * R; base += BS) {
             const int r = base + (int)threadIdx.x;
             const bool act = r < R * R;
@@ -437,7 +655,7 @@ __device__ void cull_chunk(const CullArgs &a, uint32_t k, uint32_t n, uint64_t *
                 }
             }
             __syncthreads();
-#pragma unroll
+#pragma unroll 1
             for (int l = 2; l <= 4; l++) {
                 const int nc = G::n(l), np = G::n(l - 1);
                 const int dst = L::P1 + (G::row_base(l) - G::RB1);
@@ -463,39 +681,64 @@ __device__ void cull_chunk(const CullArgs &a, uint32_t k, uint32_t n, uint64_t *
             __syncthreads();
         }
 
-        // ---- 4. row counts -> workgroup scan -> row offsets ----
-        constexpr int K = (G::ROWS + BS - 1) / BS;
-        uint32_t cnt[K];
-        uint32_t mine = 0;
-        const int r0 = (int)threadIdx.x * K;
+        // ---- 4. per-slice counts -> scan -> slice offsets ----
+        // Records are ordered by level, z, y, x, so every z-slice of every
+        // level owns a contiguous range: one offset per slice is enough.
+        uint32_t *soff = reinterpret_cast<uint32_t *>(
+            reinterpret_cast<uint8_t *>(lds + L::SCR) + L::O_SOFF);
+        {
+            const int w = (int)(threadIdx.x >> 6);
+            const uint32_t ln = lane_id();
+            for (int sl = w; sl < L::NSLICES; sl += NW) {
+                const int l = sl < L::SLB1 ? 0 : sl < L::SLB2 ? 1 : sl < L::SLB3 ? 2
+                            : sl < L::SLB4 ? 3 : 4;
+                const int nl = G::n(l);
+                const int z = sl - (l == 0 ? L::SLB0 : l == 1 ? L::SLB1 : l == 2 ? L::SLB2
+                                    : l == 3 ? L::SLB3 : L::SLB4);
+                uint32_t cv = 0;
+                for (int y = (int)ln; y < nl; y += 64) {
+                    if (l == 0) {
+                        cv += __popcll(lds[L::V0 + z * N0 + y]);
+                        if (z < R && y < R) cv += (uint32_t)(lds[L::XB + z] >> y) & 1u;
+                    } else {
+                        cv += __popcll(lds[L::P1 + (G::row_base(l) - G::RB1) + z * nl + y]);
+                    }
+                }
 #pragma unroll
-        for (int j = 0; j < K; j++) {
-            const int ro = r0 + j;
-            uint32_t cv = 0;
-            if (ro < G::RB1) {
-                const int z = ro / N0, y = ro % N0;
-                cv = __popcll(lds[L::V0 + ro]);
-                if (z < R && y < R) cv += (uint32_t)(lds[L::XB + z] >> y) & 1u;
-            } else if (ro < G::ROWS) {
-                cv = __popcll(lds[L::P1 + ro - G::RB1]);
+                for (int d = 32; d >= 1; d >>= 1) cv += __shfl_xor(cv, d, 64);
+                if (ln == 0) soff[sl] = cv;
             }
-            cnt[j] = cv;
-            mine += cv;
-        }
-        __syncthreads();   // all reads of S done before the offsets may overwrite it
-        uint32_t run = block_excl_scan<BS>(mine, wsum, total);
-        uint32_t *rowoff = reinterpret_cast<uint32_t *>(lds + L::ROWOFF);
+            __syncthreads();
+            if (w == 0) {   // exclusive scan of the slice counts in one wave
+                constexpr int KS = (L::NSLICES + 63) / 64;
+                uint32_t part = 0;
+                for (int j = 0; j < KS; j++) {
+                    const int sl = (int)ln * KS + j;
+                    if (sl < L::NSLICES) part += soff[sl];
+                }
+                uint32_t x = part;
 #pragma unroll
-        for (int j = 0; j < K; j++) {
-            const int ro = r0 + j;
-            if (ro < G::ROWS) rowoff[ro] = run;
-            run += cnt[j];
+                for (int d = 1; d < 64; d <<= 1) {
+                    const uint32_t y = __shfl_up(x, d, 64);
+                    if (ln >= (uint32_t)d) x += y;
+                }
+                uint32_t run = x - part;
+                for (int j = 0; j < KS; j++) {
+                    const int sl = (int)ln * KS + j;
+                    if (sl < L::NSLICES) {
+                        const uint32_t cv = soff[sl];
+                        soff[sl] = run;
+                        run += cv;
+                    }
+                }
+                if (ln == 63) soff[L::NSLICES] = x;
+            }
+            __syncthreads();
         }
-        __syncthreads();
-        for (int l = 0; l < 5; l++) {
-            const uint32_t b0 = rowoff[G::row_base(l)];
-            const uint32_t b1 = (l == 4) ? total : rowoff[G::row_base(l + 1)];
-            lvl[l] = b1 - b0;
+        total = uni(soff[L::NSLICES]);
+        {
+            const int slb[6] = {L::SLB0, L::SLB1, L::SLB2, L::SLB3, L::SLB4, L::NSLICES};
+            for (int l = 0; l < 5; l++) lvl[l] = soff[slb[l + 1]] - soff[slb[l]];
         }
     }
 
@@ -556,60 +799,156 @@ __device__ void cull_chunk(const CullArgs &a, uint32_t k, uint32_t n, uint64_t *
     }
     __syncthreads();
     if (trivial || a.count_only || total == 0) return;
-    const uint64_t base = *s_base / 4;
+    const uint64_t base = uni(*s_base) / 4;
     if (base + total > a.cap) return;
 
     // ---- 5. emission ----
-    const uint32_t *rowoff = reinterpret_cast<const uint32_t *>(lds + L::ROWOFF);
-    const int w = (int)(threadIdx.x >> 6), ln = (int)lane_id();
+    // The chunk is walked in z-slice pairs (2p, 2p+1).  For each pair the
+    // workgroup stages in LDS the two slices, the +x/+y neighbour voxels next
+    // to them and the three shadow-map rows the pair's records sample; scans
+    // the counts of the pair's level-0 rows and of the level-1 rows of L1
+    // slice p (whose colours come from these two slices); expands their set
+    // bits into a list of cells; and writes one record per thread, in order,
+    // with no global access but the stores.  The next pair's loads are in
+    // flight meanwhile.  Levels 2..4 follow level by level, reading colours
+    // and shadows from global memory (a few percent of the records).
+    uint8_t *scr = reinterpret_cast<uint8_t *>(lds + L::SCR);
+    const uint32_t *soff = reinterpret_cast<const uint32_t *>(scr + L::O_SOFF);
+    uint8_t *stg = scr + L::O_STG;
+    uint16_t *rowp = reinterpret_cast<uint16_t *>(scr + L::O_ROWP);
+    uint16_t *list = reinterpret_cast<uint16_t *>(scr + L::O_LIST);
     uint64_t *out = a.splats + base;
-    for (int ro = w; ro < G::ROWS; ro += NW) {
-        if (ro < G::RB1) {
-            const int z = ro / N0, y = ro % N0;
-            const uint64_t bits = lds[L::V0 + ro];
-            const uint32_t extra = (z < R && y < R) ? (uint32_t)(lds[L::XB + z] >> y) & 1u : 0u;
-            if (!bits && !extra) continue;
-            const uint32_t o = rowoff[ro];
-            if (ln < R && ((bits >> ln) & 1ull)) {
-                const uint32_t col = l0_colour<B>(c, ln, y, z);
-                const uint32_t wx = c.org[0] + ln, wy = c.org[1] + y, wz = c.org[2] + z;
-                const uint32_t sh = shadow_bit(a.map, a.sx, wx, wy, wz);
+    constexpr int NP = R / 2 + 1, n1 = G::n(1);
+    constexpr int SEGW = R < 16 ? R : 16;             // bits per expansion unit
+    constexpr int NSEG0 = R / SEGW + 1;              // level-0 row: segments + the x = R cell
+    constexpr int NSEG1 = (n1 + 15) / 16;
+    PairRegs<B, BS> pr;
+    pair_load<B, BS>(a, c, 0, pr);
+    for (int p = 0; p < NP; p++) {
+        const int z0 = 2 * p;
+        const int nrow0 = (z0 + 1 <= R ? 2 : 1) * N0;
+        const int nrows = nrow0 + (a.lod ? n1 : 0);
+        __syncthreads();                      // the previous pair is done with the scratch
+        pair_store<B, BS>(pr, stg);
+        auto row_bits = [&](int i) -> uint64_t {
+            return i < nrow0 ? lds[L::V0 + z0 * N0 + i] : lds[L::P1 + p * n1 + (i - nrow0)];
+        };
+        auto row_count = [&](int i) -> uint32_t {
+            if (i < nrow0) {
+                const int z = z0 + i / N0, y = i % N0;
+                uint32_t cv = __popcll(lds[L::V0 + z * N0 + y]);
+                if (z < R && y < R) cv += (uint32_t)(lds[L::XB + z] >> y) & 1u;
+                return cv;
+            }
+            return (uint32_t)__popcll(lds[L::P1 + p * n1 + (i - nrow0)]);
+        };
+        const uint32_t npair = scan_rows<BS>(nrows, row_count, rowp, wsum);
+        if (p + 1 < NP) pair_load<B, BS>(a, c, z0 + 2, pr);
+        const uint32_t nl0 = nrow0 < nrows ? (uint32_t)rowp[nrow0] : npair;
+        // expansion: one unit per (row, 16-bit segment)
+        const int nunits = nrow0 * NSEG0 + (nrows - nrow0) * NSEG1;
+        for (int u = threadIdx.x; u < nunits; u += BS) {
+            if (u < nrow0 * NSEG0) {
+                const int i = u / NSEG0, sg = u % NSEG0;
+                const int sidx = i / N0, y = i % N0, z = z0 + sidx;
+                const uint64_t rb = row_bits(i);
+                const uint32_t code0 = ((uint32_t)sidx << 14) | ((uint32_t)y << 7);
+                if (sg == NSEG0 - 1) {             // the cell x = R of this row
+                    if (z < R && y < R && ((lds[L::XB + z] >> y) & 1ull))
+                        list[rowp[i] + __popcll(rb)] = (uint16_t)(code0 | (uint32_t)R);
+                } else {
+                    uint32_t pos = rowp[i] + (sg ? (uint32_t)__popcll(rb & ((1ull << (sg * SEGW)) - 1ull)) : 0u);
+                    uint32_t seg = (uint32_t)(rb >> (sg * SEGW)) & ((SEGW == 32 ? 0u : (1u << SEGW)) - 1u);
+                    while (seg) {
+                        const uint32_t x = (uint32_t)(sg * SEGW) + (uint32_t)__builtin_ctz(seg);
+                        seg &= seg - 1u;
+                        list[pos++] = (uint16_t)(code0 | x);
+                    }
+                }
+            } else {
+                const int v = u - nrow0 * NSEG0, i = nrow0 + v / NSEG1, sg = v % NSEG1;
+                const int y = i - nrow0;
+                const uint64_t rb = row_bits(i);
+                uint32_t pos = rowp[i] + (sg ? (uint32_t)__popcll(rb & ((1ull << (sg * 16)) - 1ull)) : 0u);
+                uint32_t seg = (uint32_t)(rb >> (sg * 16)) & 0xFFFFu;
+                while (seg) {
+                    const uint32_t x = (uint32_t)(sg * 16) + (uint32_t)__builtin_ctz(seg);
+                    seg &= seg - 1u;
+                    list[pos++] = (uint16_t)(0x8000u | ((uint32_t)y << 7) | x);
+                }
+            }
+        }
+        __syncthreads();
+        const uint32_t o0 = soff[L::SLB0 + z0], o1 = soff[L::SLB1 + p];
+        for (uint32_t j = threadIdx.x; j < npair; j += BS) {
+            const uint32_t e = list[j];
+            const int x = (int)(e & 0x7Fu), y = (int)((e >> 7) & 0x7Fu);
+            if (j < nl0) {
+                const int z = z0 + (int)((e >> 14) & 1u);
+                const uint32_t col = staged_colour<B>(stg, x, y, z, z0);
+                const uint32_t wy = c.org[1] + y;
+                const uint32_t sh = staged_shadow<B>(stg, z - z0, x + y, wy);
                 uint32_t fl = 0;
-                if constexpr (V2) fl = face_flags<B, V2>(lds, ln, y, z);
-                out[o + rank_below(bits)] = pack_record<V2>(wx, wy, wz, col, sh, fl);
-            }
-            if (extra && ln == 0) {
-                const uint32_t col = l0_colour<B>(c, R, y, z);
-                const uint32_t wx = c.org[0] + R, wy = c.org[1] + y, wz = c.org[2] + z;
-                const uint32_t sh = shadow_bit(a.map, a.sx, wx, wy, wz);
-                out[o + __popcll(bits)] = pack_record<V2>(wx, wy, wz, col, sh, 2u);
-            }
-        } else {
-            int l = 1;
-            while (l < 4 && ro >= G::row_base(l + 1)) l++;
-            const int nl = G::n(l), idx = ro - G::row_base(l);
-            const int z = idx / nl, y = idx % nl;
-            const uint64_t bits = lds[L::P1 + ro - G::RB1];
-            if (!bits) continue;
-            if (ln < nl && ((bits >> ln) & 1ull)) {
-                int qx = ln, qy = y, qz = z;
-                descend<B, V2>(lds, l, qx, qy, qz);
-                const uint32_t col = l0_colour<B>(c, qx, qy, qz);
-                const uint32_t wx = c.org[0] + ((uint32_t)ln << l);
-                const uint32_t wy = c.org[1] + ((uint32_t)y << l);
-                const uint32_t wz = c.org[2] + ((uint32_t)z << l);
-                const uint32_t d = 1u << l;
-                const uint32_t sh = shadow_bit(a.map, a.sx, wx + d, wy + d, wz + d);
+                if constexpr (V2) fl = face_flags<B, V2>(lds, x, y, z);
+                out[o0 + j] = pack_record<V2>(c.org[0] + x, wy, c.org[2] + z, col, sh, fl);
+            } else {
+                const int yy = (int)((e >> 7) & 0x3Fu);
+                int qx = x, qy = yy, qz = p;
+                descend<B, V2>(lds, 1, qx, qy, qz);
+                const uint32_t col = staged_colour<B>(stg, qx, qy, qz, z0);
+                const uint32_t wy = c.org[1] + 2u * yy;
+                const uint32_t sh = staged_shadow<B>(stg, 2, 2 * (x + yy) + 4, wy + 2u);
                 uint32_t fl = 0;
                 if constexpr (V2) fl = face_flags<B, V2>(lds, qx, qy, qz);
-                out[rowoff[ro] + rank_below(bits)] = pack_record<V2>(wx, wy, wz, col, sh, fl);
+                out[o1 + (j - nl0)] = pack_record<V2>(c.org[0] + 2u * x, wy, c.org[2] + 2u * p, col,
+                                                      sh, fl);
             }
+        }
+    }
+    if (!a.lod) return;
+    // levels 2..4: one level at a time, (z, y, x) of each cell in the list
+    for (int l = 2; l <= 4; l++) {
+        const int nl = G::n(l), nrows = nl * nl;
+        const uint64_t *prow = lds + L::P1 + (G::row_base(l) - G::RB1);
+        __syncthreads();
+        auto row_count = [&](int i) -> uint32_t { return (uint32_t)__popcll(prow[i]); };
+        const uint32_t nlev = scan_rows<BS>(nrows, row_count, rowp, wsum);
+        for (int i = threadIdx.x; i < nrows; i += BS) {
+            uint64_t rb = prow[i];
+            uint32_t pos = rowp[i];
+            const uint32_t zy = ((uint32_t)(i / nl) << 10) | ((uint32_t)(i % nl) << 5);
+            while (rb) {
+                const uint32_t x = (uint32_t)__builtin_ctzll(rb);
+                rb &= rb - 1ull;
+                list[pos++] = (uint16_t)(zy | x);
+            }
+        }
+        __syncthreads();
+        const uint32_t ol = soff[l == 2 ? L::SLB2 : l == 3 ? L::SLB3 : L::SLB4];
+        const uint32_t d = 1u << l;
+        for (uint32_t j = threadIdx.x; j < nlev; j += BS) {
+            const uint32_t e = list[j];
+            const int x = (int)(e & 31u), y = (int)((e >> 5) & 31u), z = (int)(e >> 10);
+            int qx = x, qy = y, qz = z;
+            descend<B, V2>(lds, l, qx, qy, qz);
+            const uint32_t col = l0_colour<B>(c, qx, qy, qz);
+            const uint32_t wx = c.org[0] + ((uint32_t)x << l);
+            const uint32_t wy = c.org[1] + ((uint32_t)y << l);
+            const uint32_t wz = c.org[2] + ((uint32_t)z << l);
+            const uint32_t sh = shadow_bit(a.map, a.sx, wx + d, wy + d, wz + d);
+            uint32_t fl = 0;
+            if constexpr (V2) fl = face_flags<B, V2>(lds, qx, qy, qz);
+            out[ol + j] = pack_record<V2>(wx, wy, wz, col, sh, fl);
         }
     }
 }
 
+// LDS (~79 KiB) allows two workgroups per CU; ask the register allocator for
+// the matching occupancy (512 threads: 4 waves per SIMD, at most 128 VGPRs;
+// 1024 threads: one workgroup of 4 waves per SIMD).
 template <int B, int BS, bool V2, bool MASK_OUT>
-__global__ void __launch_bounds__(BS) k_cull(CullArgs a)
+__global__ void __launch_bounds__(BS)
+__attribute__((amdgpu_waves_per_eu(BS == 256 ? 2 : 4))) k_cull(CullArgs a)
 {
     __shared__ uint64_t lds[Lay<B, V2>::TOTAL];
     __shared__ uint32_t wsum[BS / 64 + 1];
@@ -630,7 +969,7 @@ __global__ void __launch_bounds__(BS) k_cull(CullArgs a)
     for (;;) {
         if (threadIdx.x == 0) s_ticket = atomicAdd(a.ticket, 1u);
         __syncthreads();
-        const uint32_t k = s_ticket;
+        const uint32_t k = uni(s_ticket);
         if (k >= n) break;
         cull_chunk<B, BS, V2, MASK_OUT>(a, k, n, lds, wsum, &s_base);
         __syncthreads();
