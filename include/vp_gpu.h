@@ -137,6 +137,8 @@ int vp_gpu_lod_mip(vp_ctx *ctx, uint32_t root_bitw, int level,
  * ENCODE: chunks d_chunk_ids[0..n) of d_voxels -> d_words, concatenated in
  *   list order with d_word_offsets[n+1] (in words, terminators included).
  *   d_uniform_out (nullable) receives -1 / the uniform value per listed chunk.
+ *   d_words = NULL with capacity_words = 0 sizes the streams only: offsets
+ *   (and d_uniform_out) are written, no words, no overflow is raised.
  * DECODE: d_words/d_word_offsets -> chunks d_chunk_ids[k] of d_voxels.
  *   A stream that does not expand to exactly R^3 voxels raises
  *   VP_STATUS_BAD_RLE (the chunk is written up to R^3 bytes); offsets outside

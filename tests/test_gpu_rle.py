@@ -56,7 +56,12 @@ def test_generated_world_roundtrip(gpu):
     vox = torch.empty(w.chunks * w.R ** 3, dtype=torch.uint8, device="cuda")
     gpu.generate(w, cfg.gen(), vox)
     ids = np.arange(w.chunks, dtype=np.uint32)
-    words, offs, _ = gpu.rle_encode(w, vox, ids_dev(ids), capacity=w.chunks * (w.R ** 3 + 1))
+    _, coffs, cuni = gpu.rle_encode(w, vox, ids_dev(ids), count_only=True)
+    words, offs, uni = gpu.rle_encode(w, vox, ids_dev(ids))     # sized by a counting pass
+    torch.cuda.synchronize()
+    assert gpu.status() == 0
+    assert torch.equal(coffs, offs) and torch.equal(cuni, uni)
+    assert words.numel() == int(offs[-1])
     out = torch.zeros_like(vox)
     gpu.rle_decode(w, words, offs, ids_dev(ids), out)
     torch.cuda.synchronize()

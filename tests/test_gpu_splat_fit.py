@@ -1,0 +1,136 @@
+"""Differentiable colour fitting through vp_gpu_splat / vp_gpu_splat_backward:
+torch.autograd.gradcheck of the autograd wrapper, a central-difference
+directional check on a generated terrain, the GPU gradient against the
+oracle's fp64 backward, and convergence of the Adam fit."""
+import warnings
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from tests.gpuutil import dev, host
+from voxplat_amd import workload as W
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def scene(gpu):
+    """A 128x128x128 generated world, its level-0 list and 4 views at 96x64."""
+    from voxplat_amd import splat_fit as F
+
+    cfg = W.small_world(6, (1, 1, 1), 0x5EED0044, hmax=90)
+    w, vox, splats = F.build_scene(gpu, cfg)
+    cams = [W.make_camera(*p, 96, 64) for p in
+            W.splat_poses(_top(w, vox), n=4, seed=cfg.seed + 1, lo=24.0, hi=104.0)]
+    return w, splats, cams
+
+
+def _top(w, vox):
+    R = w.R
+    vh = host(vox).reshape(1 << w.max_bitw[2], 1 << w.max_bitw[1], 1 << w.max_bitw[0], R, R, R)
+
+    def top(x, z):
+        col = vh[z // R, :, x // R, z % R, :, x % R].reshape(-1)
+        ys = np.nonzero(col)[0]
+        return int(ys.max()) + 1 if len(ys) else 0
+    return top
+
+
+def test_gradcheck_tiny_scene(gpu):
+    """Full Jacobian check on 12 splats a camera looks straight at.  The
+    backward sums pixels with float atomics, whose order varies between runs,
+    hence a small nondet_tol."""
+    import torch
+    from voxplat_amd import splat_fit as F
+
+    splats = np.array([[x, y, 0, 5] for x in range(4) for y in range(3)], np.int16)
+    sd = dev(splats)
+    cam = W.make_camera((2.0, 1.5, 9.0), 0.0, 0.0, 24, 16)
+    rgb = torch.rand((len(splats), 3), dtype=torch.float64, device="cuda").requires_grad_(True)
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore")
+        assert torch.autograd.gradcheck(lambda c: F.render(gpu, c, sd, cam), (rgb,), eps=1e-2,
+                                        atol=1e-4, rtol=1e-3, nondet_tol=1e-5)
+    img = F.render(gpu, rgb, sd, cam)
+    assert img.dtype == torch.float64 and img.shape == (16, 24, 3)
+
+
+def test_gradcheck_fast_mode_terrain(gpu, scene):
+    import torch
+    from voxplat_amd import splat_fit as F
+
+    w, splats, cams = scene
+    n = splats.shape[0]
+    rgb = torch.rand((n, 3), dtype=torch.float64, device="cuda").requires_grad_(True)
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore")
+        assert torch.autograd.gradcheck(lambda c: F.render(gpu, c, splats, cams[0]), (rgb,),
+                                        eps=1e-2, atol=1e-3, rtol=1e-3, nondet_tol=1e-5,
+                                        fast_mode=True)
+
+
+def test_directional_derivative_terrain(gpu, scene):
+    """<dL/drgb, v> against (L(rgb + h v) - L(rgb - h v)) / 2h for L = sum(G * img)."""
+    import torch
+    from voxplat_amd import splat_fit as F
+
+    w, splats, cams = scene
+    n = splats.shape[0]
+    g = torch.Generator(device="cuda").manual_seed(7)
+    rgb = torch.rand((n, 3), generator=g, device="cuda", dtype=torch.float64)
+    v = torch.randn((n, 3), generator=g, device="cuda", dtype=torch.float64)
+    for cam in cams:
+        G = torch.randn((cam.height, cam.width, 3), generator=g, device="cuda",
+                        dtype=torch.float64)
+        x = rgb.clone().requires_grad_(True)
+        (F.render(gpu, x, splats, cam) * G).sum().backward()
+        analytic = float((x.grad * v).sum())
+        h = 1e-2
+        with torch.no_grad():
+            lp = float((F.render(gpu, rgb + h * v, splats, cam) * G).sum())
+            lm = float((F.render(gpu, rgb - h * v, splats, cam) * G).sum())
+        fd = (lp - lm) / (2 * h)
+        assert abs(analytic - fd) <= 1e-3 * max(1.0, abs(fd)), (analytic, fd)
+
+
+def test_gradient_matches_oracle_backward(gpu, scene):
+    import torch
+    from voxplat_amd import splat_fit as F
+
+    w, splats, cams = scene
+    n = splats.shape[0]
+    sh = host(splats)
+    rng = np.random.default_rng(3)
+    rgb = rng.random((n, 3)).astype(np.float32)
+    cam = cams[1]
+    G = rng.standard_normal((cam.height, cam.width, 3)).astype(np.float32)
+    x = dev(rgb).requires_grad_(True)
+    img = F.render(gpu, x, splats, cam)
+    img.backward(dev(G))
+    _, eids, er, ew = O.splat_render(sh, cam, voxel_rgb=rgb, rgb=True, weight=True)
+    assert np.allclose(host(img), er, rtol=1e-5, atol=1e-5)
+    eg = O.splat_backward(n, cam, eids, ew, G)
+    assert np.allclose(host(x.grad), eg, rtol=1e-5, atol=1e-5)
+
+
+def test_adam_fit_recovers_visible_colours(gpu, scene):
+    import torch
+    from voxplat_amd import splat_fit as F
+
+    w, splats, cams = scene
+    n = splats.shape[0]
+    g = torch.Generator(device="cuda").manual_seed(11)
+    gt = torch.rand((n, 3), generator=g, device="cuda")
+    init = torch.rand((n, 3), generator=g, device="cuda")
+    with torch.no_grad():
+        targets = [F.render(gpu, gt, splats, c) for c in cams]
+    seen = F.visible(gpu, splats, cams)
+    assert int(seen.sum()) > 1000
+    rgb, hist = F.fit_colours(gpu, splats, cams, targets, init, steps=80, lr=0.05)
+    assert hist[-1] < 1e-3 * hist[0], (hist[0], hist[-1])
+    err0 = float((init[seen] - gt[seen]).abs().mean())
+    err = float((rgb[seen] - gt[seen]).abs().mean())
+    assert err < 0.05 * err0, (err0, err)
+    # colours no view sees receive no gradient and stay where they started
+    assert torch.equal(rgb[~seen], init[~seen].clamp(0, 1))

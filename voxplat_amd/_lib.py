@@ -152,19 +152,38 @@ class GPU:
         return dst
 
     # ---- RLE -----------------------------------------------------------------
-    def rle_encode(self, world: abi.WorldDesc, voxels, ids, capacity: int, uniform_out=True,
-                   stream=None):
+    def rle_encode(self, world: abi.WorldDesc, voxels, ids, capacity: int | None = None,
+                   uniform_out=True, count_only: bool = False, out=None, stream=None):
+        """Encode chunks `ids`.  Returns (words int32 or None, offsets int64[n+1]
+        in words, uniform int16[n] or None).  capacity=None sizes the streams
+        with a counting pass first; count_only returns after that pass;
+        out=(words, offsets, uniform) reuses buffers (words may be None)."""
         torch = self.torch
         n = int(ids.numel())
         dev = voxels.device
-        words = torch.empty(max(capacity, 1), dtype=torch.int32, device=dev)
-        offsets = torch.empty(n + 1, dtype=torch.int64, device=dev)
-        uni = torch.empty(n, dtype=torch.int16, device=dev) if uniform_out else None
-        _check(self.lib.vp_gpu_rle(self.ctx, abi.VP_RLE_ENCODE, C.byref(world),
-                                   C.c_void_p(voxels.data_ptr()), C.c_void_p(ids.data_ptr()), n,
-                                   C.c_void_p(words.data_ptr()), capacity,
-                                   C.c_void_p(offsets.data_ptr()), C.c_void_p(_ptr(uni)),
-                                   C.c_void_p(self._stream(stream))), "vp_gpu_rle(encode)")
+        if out is None:
+            offsets = torch.empty(n + 1, dtype=torch.int64, device=dev)
+            uni = torch.empty(n, dtype=torch.int16, device=dev) if uniform_out else None
+            words = None
+        else:
+            words, offsets, uni = out
+        st = C.c_void_p(self._stream(stream))
+
+        def run(wptr, cap, what):
+            _check(self.lib.vp_gpu_rle(self.ctx, abi.VP_RLE_ENCODE, C.byref(world),
+                                       C.c_void_p(voxels.data_ptr()), C.c_void_p(ids.data_ptr()),
+                                       n, C.c_void_p(wptr), cap, C.c_void_p(offsets.data_ptr()),
+                                       C.c_void_p(_ptr(uni)), st), what)
+        if count_only or (capacity is None and words is None):
+            run(None, 0, "vp_gpu_rle(encode, count)")
+            if count_only:
+                return None, offsets, uni
+            capacity = int(offsets[n].item()) if n else 0
+        if words is None:
+            words = torch.empty(max(capacity, 1), dtype=torch.int32, device=dev)
+        elif capacity is None:
+            capacity = int(words.numel())
+        run(words.data_ptr(), capacity, "vp_gpu_rle(encode)")
         return words, offsets, uni
 
     def rle_decode(self, world: abi.WorldDesc, words, offsets, ids, voxels, stream=None):

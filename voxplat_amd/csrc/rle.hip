@@ -203,12 +203,12 @@ __device__ void rle_encode_chunk(const RleArgs &a, uint32_t k, uint32_t n, uint6
             if (k + 1 == n) a.offsets[n] = excl + nwords;
             if (a.uniform_out)
                 a.uniform_out[k] = (runs == 1) ? (int16_t)(src ? src[0] : 0) : (int16_t)-1;
-            if (excl + nwords > a.cap) atomicOr(a.status, (uint32_t)VP_STATUS_OVERFLOW);
+            if (a.words && excl + nwords > a.cap) atomicOr(a.status, (uint32_t)VP_STATUS_OVERFLOW);
         }
     }
     __syncthreads();
     const uint64_t base = *s_base;
-    if (base + nwords > a.cap) return;
+    if (!a.words || base + nwords > a.cap) return;
 
     // ---- 4. words ----
     uint32_t *out = a.words + base;
@@ -360,9 +360,10 @@ extern "C" int vp_gpu_rle(vp_ctx *ctx, int dir, const vp_world_desc *world,
                           uint32_t *d_words, uint64_t capacity_words,
                           uint64_t *d_word_offsets, int16_t *d_uniform_out, void *stream)
 {
-    if (!ctx || !world || !d_voxels || !d_chunk_ids || !d_words || !d_word_offsets)
-        return VP_ERR_ARG;
+    if (!ctx || !world || !d_voxels || !d_chunk_ids || !d_word_offsets) return VP_ERR_ARG;
     if (dir != VP_RLE_ENCODE && dir != VP_RLE_DECODE) return VP_ERR_ARG;
+    // d_words == NULL with capacity 0: an encode that only sizes the streams
+    if (!d_words && (dir != VP_RLE_ENCODE || capacity_words != 0)) return VP_ERR_ARG;
     const uint32_t root_bitw = world->root_bitw;
     if (root_bitw < 2 || root_bitw > 6) return VP_ERR_UNSUPPORTED;
     if ((int)world->max_bitw[0] + world->max_bitw[1] + world->max_bitw[2] > 32) return VP_ERR_ARG;

@@ -1,0 +1,165 @@
+"""Differentiable splat colours: fit per-voxel fp32 RGB to target views.
+
+The forward is vp_gpu_splat with voxel colours overriding the 6-bit palette
+(the shading of resources/splat.vert.glsl:24-32 and splat.frag.glsl:93-154);
+the backward is vp_gpu_splat_backward, a scatter-add of weight * dL/drgb into
+the colour of the nearest splat of each pixel.  For a fixed hit the pixel
+colour is affine in that splat's colour:
+
+    rgb(p) = (rgb[id(p)] * s1 * s2) * (1 - fog) + fog_colour * fog
+    weight(p) = (s1 * s2) * (1 - fog)
+
+with s1, s2 the two shading factors and fog the depth fog of splat.frag, none
+of which depend on colours; the hits (ids, depths) do not either, so
+d rgb(p) / d rgb[id(p)] = weight(p) exactly and the gradient is exact up to
+fp32 rounding of the product.
+
+    python -m voxplat_amd.splat_fit        # the 1024x256x1024 colour fit
+"""
+from __future__ import annotations
+
+import json
+import sys
+import time
+
+import torch
+
+from . import abi
+from . import workload as W
+from ._lib import GPU
+
+
+class SplatColour(torch.autograd.Function):
+    """image[H, W, 3] = splat(rgb[n, 3]); gradient w.r.t. rgb only."""
+
+    @staticmethod
+    def forward(ctx, rgb, gpu: GPU, splats, cam: abi.Camera, lod=None, lod_log2: int = 0):
+        if rgb.dim() != 2 or rgb.shape[1] != 3 or rgb.shape[0] != splats.shape[0]:
+            raise ValueError(f"rgb must be [{splats.shape[0]}, 3], got {tuple(rgb.shape)}")
+        rgb_c = rgb.detach().to(torch.float32).contiguous()
+        _, ids, img, wt = gpu.splat(splats, cam, lod=lod, lod_log2=lod_log2, voxel_rgb=rgb_c,
+                                    rgb=True, weight=True)
+        ctx.save_for_backward(ids, wt)
+        ctx.gpu, ctx.cam, ctx.n, ctx.dtype = gpu, cam, rgb.shape[0], rgb.dtype
+        return img.to(rgb.dtype)
+
+    @staticmethod
+    def backward(ctx, g_img):
+        ids, wt = ctx.saved_tensors
+        grad = torch.zeros((ctx.n, 3), dtype=torch.float32, device=g_img.device)
+        ctx.gpu.splat_backward(ctx.n, ctx.cam, ids, wt, g_img.to(torch.float32).contiguous(), grad)
+        return grad.to(ctx.dtype), None, None, None, None, None
+
+
+def render(gpu: GPU, rgb, splats, cam, lod=None, lod_log2: int = 0):
+    return SplatColour.apply(rgb, gpu, splats, cam, lod, lod_log2)
+
+
+def build_scene(gpu: GPU, cfg: W.WorldConfig = W.CFG4):
+    """Generate the world on the GPU and return its level-0 splat list."""
+    w = cfg.desc()
+    dev = "cuda"
+    vox = torch.empty(w.chunks * w.R ** 3, dtype=torch.uint8, device=dev)
+    uni = torch.empty(w.chunks, dtype=torch.int16, device=dev)
+    gpu.generate(w, cfg.gen(), vox, uni)
+    sx, sz = w.shadow_dims()
+    smap = torch.empty(abi.shadow_padded_len(sx, sz), dtype=torch.int16, device=dev)
+    gpu.shadow_build(w, vox, smap)
+    ids = torch.arange(w.chunks, dtype=torch.int32, device=dev)
+    recs, offs, counts = gpu.cull_mesh(w, vox, ids, shadow_map=smap, uniform=uni, lod=False)
+    torch.cuda.synchronize()
+    n = int(offs[-1].item()) // 4
+    return w, vox, recs[:n].contiguous()
+
+
+def views(w: abi.WorldDesc, vox, n: int, size: int, seed: int):
+    X, Y, Z = w.size
+    mx, my, mz = (1 << w.max_bitw[0]), (1 << w.max_bitw[1]), (1 << w.max_bitw[2])
+    R = w.R
+    cols = vox.view(mz, my, mx, R, R, R)
+
+    def top(x, z):
+        col = cols[z // R, :, x // R, z % R, :, x % R].reshape(-1)
+        nz = torch.nonzero(col).reshape(-1)
+        return int(nz.max().item()) + 1 if nz.numel() else 0
+    poses = W.splat_poses(top, n=n, seed=seed, lo=X * 0.125, hi=X * 0.875)
+    return [W.make_camera(p, yw, pt, size, size) for p, yw, pt in poses]
+
+
+def fit_colours(gpu: GPU, splats, cams, targets, init, steps: int = 200, lr: float = 0.05,
+                log_every: int = 0):
+    """Adam on per-splat RGB from `init`, loss = mean over views of the mean
+    squared pixel error; colours are clamped to [0, 1] after each step.
+    Returns (rgb, per-step loss list)."""
+    rgb = init.detach().clone().to(torch.float32).requires_grad_(True)
+    opt = torch.optim.Adam([rgb], lr=lr, betas=(0.9, 0.999))
+    hist = torch.zeros(steps, dtype=torch.float64, device=init.device)
+    for it in range(steps):
+        opt.zero_grad(set_to_none=True)
+        for cam, tgt in zip(cams, targets):
+            img = render(gpu, rgb, splats, cam)
+            l = ((img - tgt) ** 2).mean() / len(cams)
+            l.backward()
+            hist[it] += l.detach()
+        opt.step()
+        with torch.no_grad():
+            rgb.clamp_(0.0, 1.0)
+        if log_every and it % log_every == 0:
+            print(f"step {it}: loss {float(hist[it]):.6f}", file=sys.stderr, flush=True)
+    return rgb.detach(), hist.tolist()
+
+
+def visible(gpu: GPU, splats, cams):
+    """Mask of the splats that are the nearest hit of at least one pixel."""
+    seen = torch.zeros(splats.shape[0], dtype=torch.bool, device=splats.device)
+    for c in cams:
+        _, ids, _, _ = gpu.splat(splats, c)
+        hit = ids[ids >= 0]
+        seen[hit.long()] = True
+    return seen
+
+
+def main(argv=None) -> int:
+    import argparse
+
+    ap = argparse.ArgumentParser(description="per-splat colour fit to rendered target views")
+    ap.add_argument("--views", type=int, default=64)
+    ap.add_argument("--size", type=int, default=512)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--lr", type=float, default=0.05)
+    a = ap.parse_args(argv)
+    cfg = W.CFG4
+    gpu = GPU(0)
+    t0 = time.perf_counter()
+    w, vox, splats = build_scene(gpu, cfg)
+    n = splats.shape[0]
+    cams = views(w, vox, a.views, a.size, cfg.seed + 1)
+    g = torch.Generator(device="cuda").manual_seed(cfg.seed)
+    gt = torch.rand((n, 3), generator=g, device="cuda")
+    init = torch.rand((n, 3), generator=g, device="cuda")
+    with torch.no_grad():
+        targets = [render(gpu, gt, splats, c) for c in cams]
+    seen = visible(gpu, splats, cams)
+    torch.cuda.synchronize()
+    t_setup = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    rgb, hist = fit_colours(gpu, splats, cams, targets, init, steps=a.steps, lr=a.lr,
+                            log_every=20)
+    torch.cuda.synchronize()
+    t_fit = time.perf_counter() - t0
+    print(json.dumps({
+        "workload": f"colour fit, {cfg.name}, {a.views} views at {a.size}x{a.size}, "
+                    f"Adam lr {a.lr}, {a.steps} steps",
+        "splats": n, "visible_splats": int(seen.sum()), "steps": a.steps,
+        "ms_per_step": round(1e3 * t_fit / a.steps, 3),
+        "renders_per_s": round(a.steps * a.views / t_fit, 1),
+        "loss_first": hist[0], "loss_last": hist[-1],
+        "mean_abs_rgb_error_visible_init": round(float((init[seen] - gt[seen]).abs().mean()), 5),
+        "mean_abs_rgb_error_visible": round(float((rgb[seen] - gt[seen]).abs().mean()), 5),
+        "setup_s": round(t_setup, 2)}))
+    gpu.close()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())
