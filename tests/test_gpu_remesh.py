@@ -49,6 +49,23 @@ def test_random_worlds_batched(gpu, rb, mb, density, seed):
         _check(w, vox, [c], smap, gpu_remesh(gpu, w, vox, [c], smap))
 
 
+@pytest.mark.parametrize("block,wg", [(256, 2), (1024, 1), (512, 2)])
+def test_launch_shapes(gpu, block, wg):
+    """Every workgroup size the autotuner may pick gives the same bytes
+    (the last case restores the default shape)."""
+    cfg = W.small_world(6, (2, 1, 2), 0x5EED0077, hmax=100)
+    w = cfg.desc()
+    vox = O.gen_world(w, cfg.gen())
+    smap = O.shadow_build(w, vox)
+    ids = np.arange(w.chunks, dtype=np.uint32)
+    gpu.set_launch(block, wg)
+    try:
+        got = gpu_remesh(gpu, w, vox, ids, smap)
+    finally:
+        gpu.set_launch(512, 2)
+    _check(w, vox, ids, smap, got)
+
+
 @pytest.mark.parametrize("rb", [3, 6])
 def test_generated_world_with_uniform_chunks(gpu, rb):
     # heights well inside the world: all-air chunks above, all-solid below

@@ -1,0 +1,70 @@
+"""Plain launches of the batched remesh on a generated world, for profilers.
+
+    python -m voxplat_amd.remesh_run [--world 2k|16k] [--launches K] [--copy-gib G]
+
+Generates the world on the GPU, sizes the output with a counting pass, then
+issues K direct (not graph-captured) launches of vp_gpu_cull_mesh over every
+chunk, so that a kernel tracer or counter collector sees one dispatch per
+launch.  --copy-gib G first runs one device-to-device copy of G GiB: a kernel
+of known traffic (G GiB read, G GiB written) against which HBM byte counters
+are scaled.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import sys
+
+import torch
+
+from . import abi
+from . import workload as W
+from ._lib import GPU
+
+
+def main(argv=None) -> int:
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--world", choices=("2k", "16k"), default="2k")
+    ap.add_argument("--launches", type=int, default=3)
+    ap.add_argument("--copy-gib", type=float, default=0.0)
+    a = ap.parse_args(argv)
+    gpu = GPU(0)
+    if a.copy_gib:
+        n = int(a.copy_gib * 2 ** 30)
+        src = torch.ones(n, dtype=torch.uint8, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        print(f"copy: {n} bytes", file=sys.stderr, flush=True)
+        del src, dst
+    cfg = W.CFG1 if a.world == "2k" else W.CFG3
+    w = cfg.desc()
+    vox = torch.empty(w.chunks * w.R ** 3, dtype=torch.uint8, device="cuda")
+    uni = torch.empty(w.chunks, dtype=torch.int16, device="cuda")
+    gpu.generate(w, cfg.gen(), vox, uni)
+    sx, sz = w.shadow_dims()
+    smap = torch.empty(abi.shadow_padded_len(sx, sz), dtype=torch.int16, device="cuda")
+    gpu.shadow_build(w, vox, smap)
+    ids = torch.arange(w.chunks, dtype=torch.int32, device="cuda")
+    _, offs, counts = gpu.cull_mesh(w, vox, ids, shadow_map=smap, uniform=uni, count_only=True)
+    torch.cuda.synchronize()
+    cap = int(offs[-1].item()) // 4
+    splats = torch.empty((max(cap, 1), 4), dtype=torch.int16, device="cuda")
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    ms = []
+    for _ in range(a.launches):
+        e0.record()
+        gpu.cull_mesh(w, vox, ids, shadow_map=smap, uniform=uni, capacity=cap,
+                      out=(offs, counts))
+        e1.record()
+        e1.synchronize()
+        ms.append(e0.elapsed_time(e1))
+    st = gpu.status()
+    print(json.dumps({"world": cfg.name, "chunks": w.chunks, "records": cap,
+                      "launch_ms": [round(x, 4) for x in ms], "status": st}))
+    gpu.close()
+    return 0 if st == 0 else 1
+
+
+if __name__ == "__main__":
+    sys.exit(main())

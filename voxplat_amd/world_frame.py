@@ -230,9 +230,12 @@ def main(argv=None) -> int:
 
 
 def run_world16k(frames: int = 5, warmup: int = 1, edits: int = 10000, log=None,
-                 keep: bool = False, gpu: GPU | None = None, full_steps: int = 0):
+                 keep: bool = False, gpu: GPU | None = None, full_steps: int = 0,
+                 on_generated=None):
     """The 16384x512x16384 workload.  Returns a dict of timings (and, with
-    keep, the (world, frame, spheres) objects under "_objects")."""
+    keep, the (world, frame, spheres) objects under "_objects").
+    on_generated(world), if given, runs after generation and the full
+    remesh, before the edits; its return value is kept under "on_generated"."""
     def say(m):
         if log:
             log(m)
@@ -251,6 +254,7 @@ def run_world16k(frames: int = 5, warmup: int = 1, edits: int = 10000, log=None,
     full = full_remesh(world, steps=full_steps, log=say) if full_steps else None
     if full:
         say(f"full remesh: {full}")
+    extra = on_generated(world) if on_generated else None
     spheres = W.edit_burst(world.w, n=edits)
     t0 = time.perf_counter()
     world.edit(spheres)
@@ -300,6 +304,7 @@ def run_world16k(frames: int = 5, warmup: int = 1, edits: int = 10000, log=None,
         "frame_gb_s": round((rb + lb) / ((rm + rl) * 1e-3) / 1e9, 1),
         "status": status,
         "full_remesh": full,
+        "on_generated": extra,
     }
     if keep:
         out["_objects"] = (world, fr, spheres)
