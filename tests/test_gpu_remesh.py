@@ -203,3 +203,34 @@ def test_cfg1_full_world_batched(gpu):
     ids = np.arange(w.chunks, dtype=np.uint32)
     got = gpu_remesh(gpu, w, vox, ids, smap, uniform=uni)
     _check(w, vox_h, ids, smap_h, got)
+
+
+def test_output_past_4gib(gpu):
+    """A list whose output passes 2^31 int16 items (4 GiB of records): the
+    2048x256x2048 world listed three times over; every copy of a chunk's
+    records must be identical and sit at its offset."""
+    import torch
+
+    cfg = W.CFG1
+    w = cfg.desc()
+    vox = torch.empty(w.chunks * w.R ** 3, dtype=torch.uint8, device="cuda")
+    uni = torch.empty(w.chunks, dtype=torch.int16, device="cuda")
+    gpu.generate(w, cfg.gen(), vox, uni)
+    sx, sz = w.shadow_dims()
+    smap = torch.empty(abi.shadow_padded_len(sx, sz), dtype=torch.int16, device="cuda")
+    gpu.shadow_build(w, vox, smap)
+    one = torch.arange(w.chunks, dtype=torch.int32, device="cuda")
+    ids = torch.cat([one, one, one])
+    recs, offs, counts = gpu.cull_mesh(w, vox, ids, shadow_map=smap, uniform=uni)
+    torch.cuda.synchronize()
+    assert gpu.status() == 0
+    n1 = int(offs[w.chunks]) // 4
+    assert int(offs[-1]) == 3 * 4 * n1 and 4 * 3 * n1 > 2 ** 31
+    items = counts.sum(1).to(torch.int64)
+    assert torch.equal(offs[1:] - offs[:-1], items)
+    first = recs[:n1]
+    assert bool((first != 0).any(dim=1).all())           # every record has a non-zero field
+    assert torch.equal(recs[n1:2 * n1], first)
+    assert torch.equal(recs[2 * n1:3 * n1], first)
+    del recs
+    torch.cuda.empty_cache()

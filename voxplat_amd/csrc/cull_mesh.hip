@@ -137,13 +137,19 @@ __device__ __forceinline__ uint32_t nz16(uint4 v)
 
 __device__ __forceinline__ uint32_t lane_id() { return threadIdx.x & 63u; }
 
-// wave-uniform values read from LDS: keep them in scalar registers
-__device__ __forceinline__ uint32_t uni(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
+// wave-uniform values read from LDS: keep them in scalar registers.  The
+// builtin returns int: each half goes through uint32_t so that a low word
+// >= 2^31 is not sign-extended into the high word.
+__device__ __forceinline__ uint32_t uni(uint32_t v)
+{
+    return (uint32_t)__builtin_amdgcn_readfirstlane(v);
+}
 __device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
 __device__ __forceinline__ uint64_t uni(uint64_t v)
 {
-    return ((uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)(v >> 32)) << 32) |
-           __builtin_amdgcn_readfirstlane((uint32_t)v);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)v);
+    return ((uint64_t)hi << 32) | (uint64_t)lo;
 }
 
 __device__ __forceinline__ uint32_t rank_below(uint64_t m)

@@ -117,20 +117,40 @@ __device__ void rle_encode_chunk(const RleArgs &a, uint32_t k, uint32_t n, uint6
     const uint8_t *src = chunk == 0xFFFFFFFFu ? nullptr : a.voxels + ((size_t)chunk << (3 * B));
 
     // ---- 1. start bits ----
+    // DEPTH 16-byte loads per thread in flight; the byte before a piece is
+    // the previous lane's last byte, except at lane 0, which loads it.
     if constexpr (VOX >= 16) {
+        constexpr int PIECES = VOX / 16;
+        constexpr int PER = PIECES / BS;
+        constexpr int DEPTH = PER >= 8 ? 8 : (PER >= 1 ? PER : 1);
         uint16_t *b16 = reinterpret_cast<uint16_t *>(bm);
-        for (int s = threadIdx.x; s < VOX / 16; s += BS) {
-            if (!src) {
-                b16[s] = s ? 0 : 1;
-                continue;
+        const uint32_t ln = r_lane();
+        if (!src) {
+            for (int s = threadIdx.x; s < PIECES; s += BS) b16[s] = s ? 0 : 1;
+        } else {
+#pragma unroll 1
+            for (int s0 = 0; s0 < PIECES; s0 += BS * DEPTH) {
+                uint4 v[DEPTH];
+                uint32_t pb[DEPTH];
+#pragma unroll
+                for (int d = 0; d < DEPTH; d++) {
+                    const int sp = s0 + d * BS + (int)threadIdx.x;
+                    const int sc = sp < PIECES ? sp : 0;
+                    v[d] = *reinterpret_cast<const uint4 *>(src + (size_t)sc * 16);
+                    pb[d] = (ln == 0u && sp > 0 && sp < PIECES) ? src[(size_t)sp * 16 - 1] : 0u;
+                }
+#pragma unroll
+                for (int d = 0; d < DEPTH; d++) {
+                    const int sp = s0 + d * BS + (int)threadIdx.x;
+                    uint32_t prev = (uint32_t)__shfl_up((int)(v[d].w >> 24), 1, 64);
+                    if (ln == 0u) prev = sp ? pb[d] : (~v[d].x & 0xFFu);
+                    uint32_t m = diff4(v[d].x, prev);
+                    m |= diff4(v[d].y, v[d].x >> 24) << 4;
+                    m |= diff4(v[d].z, v[d].y >> 24) << 8;
+                    m |= diff4(v[d].w, v[d].z >> 24) << 12;
+                    if (sp < PIECES) b16[sp] = (uint16_t)m;
+                }
             }
-            const uint4 v = *reinterpret_cast<const uint4 *>(src + (size_t)s * 16);
-            const uint32_t prev = s ? src[(size_t)s * 16 - 1] : (uint32_t)(~v.x & 0xFFu);
-            uint32_t m = diff4(v.x, prev);
-            m |= diff4(v.y, v.x >> 24) << 4;
-            m |= diff4(v.z, v.y >> 24) << 8;
-            m |= diff4(v.w, v.z >> 24) << 12;
-            b16[s] = (uint16_t)m;
         }
     }
     __syncthreads();
@@ -211,10 +231,10 @@ __device__ void rle_encode_chunk(const RleArgs &a, uint32_t k, uint32_t n, uint6
     if (!a.words || base + nwords > a.cap) return;
 
     // ---- 4. words ----
+    // adjacent lanes take adjacent bitmap words, so a store instruction's
+    // addresses stay within a few cache lines
     uint32_t *out = a.words + base;
-    for (int j = 0; j < WPT; j++) {
-        const int w = w0 + j;
-        if (w >= WORDS) break;
+    for (int w = threadIdx.x; w < WORDS; w += BS) {
         uint64_t b = bm[w];
         uint32_t o = woff[w];
         while (b) {
