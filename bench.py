@@ -151,8 +151,12 @@ def main() -> int:
     sx, sz = w.shadow_dims()
     smap = torch.empty(abi.shadow_padded_len(sx, sz), dtype=torch.int16, device=dev)
     gpu.shadow_build(w, vox, smap)
-    ids = torch.arange(nch, dtype=torch.int32, device=dev)
-    _, offs, counts = gpu.cull_mesh(w, vox, ids, shadow_map=smap, uniform=uni, count_only=True)
+    # the list is ordered heaviest chunk first (the output follows the list;
+    # every chunk's records are found through its offset)
+    from voxplat_amd.world_frame import heaviest_first
+
+    ids, offs, counts = heaviest_first(gpu, w, vox, torch.arange(nch, dtype=torch.int32, device=dev),
+                                       smap, uni)
     torch.cuda.synchronize()
     total_items = int(offs[-1].item())
     cap = total_items // 4
@@ -222,8 +226,14 @@ def main() -> int:
     # ---- splat resolve at 1080p (level-0 list, chunk order) ----
     splat_res = None
     if not args.no_splat:
-        lvl = np.repeat(np.tile(np.arange(5, dtype=np.uint8), nch), (counts_h.reshape(-1) // 4))
-        l0_idx = torch.from_numpy(np.flatnonzero(lvl == 0)).to(dev)
+        # the level-0 records in chunk-index order
+        ids_h = ids.cpu().numpy()
+        starts = (offs_h[:-1] // 4).astype(np.int64)
+        lens = (counts_h[:, 0] // 4).astype(np.int64)
+        by_chunk = np.argsort(ids_h, kind="stable")
+        starts, lens = starts[by_chunk], lens[by_chunk]
+        first = np.repeat(starts - np.concatenate([[0], np.cumsum(lens)[:-1]]), lens)
+        l0_idx = torch.from_numpy(first + np.arange(int(lens.sum()), dtype=np.int64)).to(dev)
         l0 = splats.index_select(0, l0_idx).contiguous()
         n0 = int(l0.shape[0])
         vh_cols = vox.view(32, 4, 32, 64, 64, 64)          # [cz, cy, cx, z, y, x]

@@ -94,19 +94,13 @@ struct Lay {
     static constexpr int ST_BYTES = 2 * R * R + 4 * R + 3 * SW * 2;
     static constexpr int maxi(int a, int b) { return a > b ? a : b; }
     static constexpr int al16(int v) { return (v + 15) & ~15; }
-    // rows of a slice pair's L0 slices and one L1 slice; rows of L2..L4
-    static constexpr int PR_ROWS = 2 * N0 + G::n(1);
-    static constexpr int NR24 = G::ROWS - G::RB2;
-    // cells of a slice pair's L0 slices and one L1 slice
-    static constexpr int LIST_N = 2 * N0 * N0 + G::n(1) * G::n(1);
-    // L2..L4 cells are listed one batch (at most 1024 cells) per slice pair
-    static constexpr int BATCH_MAX = 1024;
+    static constexpr int PR_ROWS = maxi(2 * N0 + G::n(1), G::n(2) * G::n(2));
+    static constexpr int LIST_N = maxi(2 * N0 * N0 + G::n(1) * G::n(1),
+                                       G::n(2) * G::n(2) * G::n(2));
     static constexpr int O_STG = 0;
     static constexpr int O_SOFF = al16(ST_BYTES);
     static constexpr int O_ROWP = O_SOFF + al16((NSLICES + 1) * 4);
-    static constexpr int O_ROWP24 = O_ROWP + al16(PR_ROWS * 2);
-    static constexpr int O_BLIST = O_ROWP24 + al16(NR24 * 2);
-    static constexpr int O_LIST = O_BLIST + al16(BATCH_MAX * 2);
+    static constexpr int O_LIST = O_ROWP + al16(PR_ROWS * 2);
     static constexpr int SCR_BYTES = O_LIST + al16(LIST_N * 2);
     static constexpr bool ALIAS = !V2 && SCR_BYTES <= R * R * 8;
     static constexpr int SCR = ALIAS ? S : PEND;
@@ -185,11 +179,16 @@ struct ChunkRefs {
 
 // Buffer resource over [p, p + bytes): loads share one VGPR offset, take the
 // per-load step from a scalar offset, and read 0 past the end.  0x00020000 is
-// the gfx9 word-3 format of a raw byte buffer.
+// the gfx9 word-3 format of a raw byte buffer.  p must be wave-uniform: it is
+// moved to scalar registers here, so the descriptor is scalar and a load
+// needs no per-lane loop over descriptors.
 __device__ __forceinline__ __amdgpu_buffer_rsrc_t byte_rsrc(const void *p, uint32_t bytes)
 {
-    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), (short)0, (int)bytes,
-                                             0x00020000);
+    const uint64_t a = (uint64_t)(uintptr_t)p;
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)a);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)(a >> 32));
+    void *u = (void *)(uintptr_t)(((uint64_t)hi << 32) | lo);
+    return __builtin_amdgcn_make_buffer_rsrc(u, (short)0, (int)bytes, 0x00020000);
 }
 
 __device__ __forceinline__ uint4 ld16(__amdgpu_buffer_rsrc_t rs, uint32_t voff, uint32_t soff)
@@ -383,8 +382,7 @@ __device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t *wsum, 
 }
 
 // ---- emission helpers --------------------------------------------------------
-// Registers carrying one slice pair from global memory to the LDS staging:
-// raw loaded values only, so the loads stay in flight until pair_store.
+// Registers carrying one slice pair from global memory to the LDS staging.
 template <int B, int BS>
 struct PairRegs {
     static constexpr int R = 1 << B;
@@ -397,7 +395,6 @@ struct PairRegs {
     uint4 sl[KSL];
     uint32_t xy[KXY];
     uint32_t sh[KSH];
-    int z0;
 };
 
 __device__ __forceinline__ uint4 splat4(uint32_t v)
@@ -406,99 +403,88 @@ __device__ __forceinline__ uint4 splat4(uint32_t v)
     return make_uint4(w, w, w, w);
 }
 
-// Issue the loads of pair (z0, z0+1); slice z = R is the +z neighbour's z=0
-// plane.  Every item is loaded unconditionally from a valid address; which
-// value it stands for (uniform chunk, null chunk, slice past R) is decided in
-// pair_store, so the loads are independent and stay in flight together.
+// Issue the loads of pair (z0, z0+1); slice z = R is the +z neighbour's z=0 plane.
 template <int B, int BS>
 __device__ __forceinline__ void pair_load(const CullArgs &a, const ChunkRefs &c, int z0,
                                           PairRegs<B, BS> &pr)
 {
     constexpr int R = 1 << B, SL = R * R;
     using P = PairRegs<B, BS>;
-    const uint8_t *nbz = c.nb[2] ? c.nb[2] : c.own;
-    const uint8_t *nbx = c.nb[0] ? c.nb[0] : c.own;
-    const uint8_t *nby = c.nb[1] ? c.nb[1] : c.own;
-    pr.z0 = z0;
 #pragma unroll
     for (int j = 0; j < P::KSL; j++) {
-        const int i0 = (int)threadIdx.x + j * BS;
-        const int i = i0 < P::NSL ? i0 : 0;
-        if constexpr (SL >= 16) {
+        const int i = (int)threadIdx.x + j * BS;
+        uint4 v = make_uint4(0, 0, 0, 0);
+        if (i < P::NSL) {
             const int byte = i * 16, sidx = byte / SL, off = byte % SL, z = z0 + sidx;
-            const uint8_t *src = z < R ? c.own + (size_t)z * SL + off : nbz + off;
-            pr.sl[j] = *reinterpret_cast<const uint4 *>(src);
-        } else {   // slices smaller than 16 bytes (R = 2): byte by byte
-            uint32_t wv[4] = {0, 0, 0, 0};
-            for (int q = 0; q < 16; q++) {
-                const int bb = i * 16 + q, zz = z0 + bb / SL, oo = bb % SL;
-                uint32_t x = 0;
-                if (zz < R) x = c.own_u >= 0 ? (uint32_t)c.own_u : c.own[(size_t)zz * SL + oo];
-                else if (zz == R) x = c.nb_u[2] >= 0 ? (uint32_t)c.nb_u[2] : c.nb[2][oo];
-                wv[q >> 2] |= (x & 0xFFu) << (8 * (q & 3));
+            if (SL >= 16) {
+                if (z < R) {
+                    v = c.own_u >= 0 ? splat4((uint32_t)c.own_u)
+                                     : *reinterpret_cast<const uint4 *>(c.own + (size_t)z * SL + off);
+                } else if (z == R) {
+                    v = c.nb_u[2] >= 0 ? splat4((uint32_t)c.nb_u[2])
+                                       : *reinterpret_cast<const uint4 *>(c.nb[2] + off);
+                }
+            } else {   // slices smaller than 16 bytes (R = 2): byte by byte
+                uint32_t wv[4] = {0, 0, 0, 0};
+                for (int q = 0; q < 16; q++) {
+                    const int bb = byte + q, zz = z0 + bb / SL, oo = bb % SL;
+                    uint32_t x = 0;
+                    if (zz < R) x = c.own_u >= 0 ? (uint32_t)c.own_u : c.own[(size_t)zz * SL + oo];
+                    else if (zz == R) x = c.nb_u[2] >= 0 ? (uint32_t)c.nb_u[2] : c.nb[2][oo];
+                    wv[q >> 2] |= (x & 0xFFu) << (8 * (q & 3));
+                }
+                v = make_uint4(wv[0], wv[1], wv[2], wv[3]);
             }
-            pr.sl[j] = make_uint4(wv[0], wv[1], wv[2], wv[3]);
         }
+        pr.sl[j] = v;
     }
 #pragma unroll
     for (int j = 0; j < P::KXY; j++) {
-        const int i0 = (int)threadIdx.x + j * BS;
-        const int i = i0 < P::NXY ? i0 : 0;
-        const int kind = i / (2 * R), sidx = (i % (2 * R)) / R, e = i % R;
-        const int z = min(z0 + sidx, R - 1);
-        // kind 0: voxel (0, e, z) of the +x neighbour; kind 1: voxel (e, 0, z) of +y
-        const uint8_t *src = kind == 0 ? nbx + (size_t)(z * R + e) * R : nby + (size_t)(z * R) * R + e;
-        pr.xy[j] = *src;
+        const int i = (int)threadIdx.x + j * BS;
+        uint32_t v = 0;
+        if (i < P::NXY) {
+            const int kind = i / (2 * R), sidx = (i % (2 * R)) / R, e = i % R, z = z0 + sidx;
+            if (z < R) {
+                if (kind == 0)   // voxel (0, e, z) of the +x neighbour
+                    v = c.nb_u[0] >= 0 ? (uint32_t)c.nb_u[0] : c.nb[0][(size_t)(z * R + e) * R];
+                else             // voxel (e, 0, z) of the +y neighbour
+                    v = c.nb_u[1] >= 0 ? (uint32_t)c.nb_u[1] : c.nb[1][(size_t)(z * R) * R + e];
+            }
+        }
+        pr.xy[j] = v;
     }
 #pragma unroll
     for (int j = 0; j < P::KSH; j++) {
-        const int i0 = (int)threadIdx.x + j * BS;
-        const int i = i0 < P::NSH ? i0 : 0;
-        const int SWn = 2 * R + 8, row = i / SWn, e = i % SWn;
-        const uint16_t *m = a.map ? a.map : reinterpret_cast<const uint16_t *>(c.own);
-        const size_t idx = a.map ? (size_t)a.sx * (c.org[2] + z0 + row) + (c.org[0] + c.org[1]) + e : 0;
-        pr.sh[j] = m[idx];
+        const int i = (int)threadIdx.x + j * BS;
+        uint32_t v = 0;
+        if (i < P::NSH && a.map) {
+            const int SWn = 2 * R + 8, row = i / SWn, e = i % SWn;
+            v = a.map[(size_t)a.sx * (c.org[2] + z0 + row) + (c.org[0] + c.org[1]) + e];
+        }
+        pr.sh[j] = v;
     }
 }
 
 template <int B, int BS>
-__device__ __forceinline__ void pair_store(const CullArgs &a, const ChunkRefs &c,
-                                           const PairRegs<B, BS> &pr, uint8_t *stg)
+__device__ __forceinline__ void pair_store(const PairRegs<B, BS> &pr, uint8_t *stg)
 {
-    constexpr int R = 1 << B, SL = R * R;
+    constexpr int R = 1 << B;
     using P = PairRegs<B, BS>;
     constexpr int XC = 2 * R * R, SH = XC + 4 * R;
-    const int z0 = pr.z0;
 #pragma unroll
     for (int j = 0; j < P::KSL; j++) {
         const int i = (int)threadIdx.x + j * BS;
-        if (i < P::NSL) {
-            uint4 v = pr.sl[j];
-            if constexpr (SL >= 16) {
-                const int z = z0 + (i * 16) / SL;
-                const int u = z < R ? c.own_u : (z == R ? c.nb_u[2] : 0);
-                if (z > R) v = make_uint4(0, 0, 0, 0);
-                else if (u >= 0) v = splat4((uint32_t)u);
-            }
-            *reinterpret_cast<uint4 *>(stg + i * 16) = v;
-        }
+        if (i < P::NSL) *reinterpret_cast<uint4 *>(stg + i * 16) = pr.sl[j];
     }
 #pragma unroll
     for (int j = 0; j < P::KXY; j++) {
         const int i = (int)threadIdx.x + j * BS;
-        if (i < P::NXY) {
-            const int kind = i / (2 * R), sidx = (i % (2 * R)) / R;
-            const int u = kind == 0 ? c.nb_u[0] : c.nb_u[1];
-            uint32_t v = pr.xy[j];
-            if (z0 + sidx >= R) v = 0;
-            else if (u >= 0) v = (uint32_t)u;
-            stg[XC + i] = (uint8_t)v;
-        }
+        if (i < P::NXY) stg[XC + i] = (uint8_t)pr.xy[j];
     }
 #pragma unroll
     for (int j = 0; j < P::KSH; j++) {
         const int i = (int)threadIdx.x + j * BS;
-        if (i < P::NSH) reinterpret_cast<uint16_t *>(stg + SH)[i] = a.map ? (uint16_t)pr.sh[j] : 0;
+        if (i < P::NSH) reinterpret_cast<uint16_t *>(stg + SH)[i] = (uint16_t)pr.sh[j];
     }
 }
 
@@ -523,10 +509,10 @@ __device__ __forceinline__ uint32_t staged_shadow(const uint8_t *stg, int row, i
     return !(rw[e] <= wy && rw[e + 1] <= wy);
 }
 
-// Exclusive scan of the counts of rows [0, nrows) into rowp (u16); each
-// thread sums a contiguous block of rows.  Ends synchronised.
+// Exclusive scan of the counts of rows [0, nrows) into rowp (u16, relative
+// to row 0); each thread sums a contiguous block of rows.  Returns the total.
 template <int BS, typename F>
-__device__ __forceinline__ void scan_rows(int nrows, F count, uint16_t *rowp, uint32_t *wsum)
+__device__ __forceinline__ uint32_t scan_rows(int nrows, F count, uint16_t *rowp, uint32_t *wsum)
 {
     const int K = (nrows + BS - 1) / BS;
     const int r0 = (int)threadIdx.x * K;
@@ -541,6 +527,7 @@ __device__ __forceinline__ void scan_rows(int nrows, F count, uint16_t *rowp, ui
             run += count(r0 + j);
         }
     __syncthreads();
+    return tot;
 }
 
 // Work for one chunk; all threads of the workgroup call it.
@@ -761,11 +748,8 @@ __device__ void cull_chunk(const CullArgs &a, uint32_t k, uint32_t n, uint64_t *
         }
         total = uni(soff[L::NSLICES]);
         {
-            lvl[0] = uni(soff[L::SLB1] - soff[L::SLB0]);
-            lvl[1] = uni(soff[L::SLB2] - soff[L::SLB1]);
-            lvl[2] = uni(soff[L::SLB3] - soff[L::SLB2]);
-            lvl[3] = uni(soff[L::SLB4] - soff[L::SLB3]);
-            lvl[4] = uni(soff[L::NSLICES] - soff[L::SLB4]);
+            const int slb[6] = {L::SLB0, L::SLB1, L::SLB2, L::SLB3, L::SLB4, L::NSLICES};
+            for (int l = 0; l < 5; l++) lvl[l] = soff[slb[l + 1]] - soff[slb[l]];
         }
     }
 
@@ -819,7 +803,6 @@ __device__ void cull_chunk(const CullArgs &a, uint32_t k, uint32_t n, uint64_t *
             *s_base = excl;
             a.offsets[k] = excl;
             if (k + 1 == n) a.offsets[n] = excl + items;
-#pragma unroll
             for (int l = 0; l < 5; l++) a.level_counts[(size_t)k * 5 + l] = 4u * lvl[l];
             if (!a.count_only && total && excl / 4 + total > a.cap)
                 atomicOr(a.status, (uint32_t)VP_STATUS_OVERFLOW);
@@ -833,59 +816,34 @@ __device__ void cull_chunk(const CullArgs &a, uint32_t k, uint32_t n, uint64_t *
     // ---- 5. emission ----
     // The chunk is walked in z-slice pairs (2p, 2p+1).  For each pair the
     // workgroup stages in LDS the two slices, the +x/+y neighbour voxels next
-    // to them and the three shadow-map rows the pair's records sample; one
-    // wave per slice scans the record counts of the pair's level-0 rows and
-    // of the level-1 rows of L1 slice p (whose colours come from these two
-    // slices); the set bits are expanded into a list of cells; and each
-    // thread writes records of consecutive list entries, so the stores are
-    // contiguous.  No global reads but the next pair's prefetch, in flight
-    // meanwhile.  Levels 2..4 (a few percent of the records) ride along in
-    // batches of one record per thread: a batch's cells are listed during one
-    // pair, their voxel and shadow-map reads are issued before that pair's
-    // stores, and the records are written after the next pair is staged, so
-    // the latency of those gathers hides behind the pair loop.
+    // to them and the three shadow-map rows the pair's records sample; scans
+    // the counts of the pair's level-0 rows and of the level-1 rows of L1
+    // slice p (whose colours come from these two slices); expands their set
+    // bits into a list of cells; and writes one record per thread, in order,
+    // with no global access but the stores.  The next pair's loads are in
+    // flight meanwhile.  Levels 2..4 follow level by level, reading colours
+    // and shadows from global memory (a few percent of the records).
     uint8_t *scr = reinterpret_cast<uint8_t *>(lds + L::SCR);
     const uint32_t *soff = reinterpret_cast<const uint32_t *>(scr + L::O_SOFF);
     uint8_t *stg = scr + L::O_STG;
     uint16_t *rowp = reinterpret_cast<uint16_t *>(scr + L::O_ROWP);
-    uint16_t *rowp24 = reinterpret_cast<uint16_t *>(scr + L::O_ROWP24);
-    uint16_t *blist = reinterpret_cast<uint16_t *>(scr + L::O_BLIST);
     uint16_t *list = reinterpret_cast<uint16_t *>(scr + L::O_LIST);
     uint64_t *out = a.splats + base;
     constexpr int NP = R / 2 + 1, n1 = G::n(1);
     constexpr int SEGW = R < 16 ? R : 16;             // bits per expansion unit
     constexpr int NSEG0 = R / SEGW + 1;              // level-0 row: segments + the x = R cell
     constexpr int NSEG1 = (n1 + 15) / 16;
-    constexpr int NR = L::NR24;
-    constexpr int BT = BS < L::BATCH_MAX ? BS : L::BATCH_MAX;   // L2..L4 records per batch
-    const uint64_t *prow = lds + L::P1 + (G::RB2 - G::RB1);
-    const uint32_t o2 = soff[L::SLB2];
-    const uint32_t n24 = a.lod ? soff[L::NSLICES] - o2 : 0u;
-    const int nb24 = (int)((n24 + (uint32_t)BT - 1u) / (uint32_t)BT);
-    if (nb24) {   // list positions of the L2..L4 rows (records follow row order)
-        auto cnt24 = [&](int i) -> uint32_t { return (uint32_t)__popcll(prow[i]); };
-        scan_rows<BS>(NR, cnt24, rowp24, wsum);
-    }
-    // this thread's pending L2..L4 record: position, colour, the two map entries
-    uint32_t q_x = 0, q_y = 0, q_z = 0, q_fl = 0, q_col = 0, q_m0 = 0, q_m1 = 0, q_wyd = 0;
-    uint32_t q_j = 0xFFFFFFFFu;
     PairRegs<B, BS> pr;
     pair_load<B, BS>(a, c, 0, pr);
     for (int p = 0; p < NP; p++) {
         const int z0 = 2 * p;
-        const int nsl0 = z0 + 1 <= R ? 2 : 1;
-        const int nrow0 = nsl0 * N0;
+        const int nrow0 = (z0 + 1 <= R ? 2 : 1) * N0;
         const int nrows = nrow0 + (a.lod ? n1 : 0);
-        const uint32_t o0 = soff[L::SLB0 + z0], o1 = soff[L::SLB1 + p];
-        const uint32_t nl0 = soff[L::SLB0 + min(z0 + 2, N0)] - o0;
-        const uint32_t npair = nl0 + (a.lod ? soff[L::SLB1 + p + 1] - o1 : 0u);
         __syncthreads();                      // the previous pair is done with the scratch
-        pair_store<B, BS>(a, c, pr, stg);
-        if (q_j != 0xFFFFFFFFu) {             // the previous batch's L2..L4 record
-            const uint32_t sh = a.map ? (uint32_t)!(q_m0 <= q_wyd && q_m1 <= q_wyd) : 0u;
-            out[o2 + q_j] = pack_record<V2>(q_x, q_y, q_z, q_col, sh, q_fl);
-            q_j = 0xFFFFFFFFu;
-        }
+        pair_store<B, BS>(pr, stg);
+        auto row_bits = [&](int i) -> uint64_t {
+            return i < nrow0 ? lds[L::V0 + z0 * N0 + i] : lds[L::P1 + p * n1 + (i - nrow0)];
+        };
         auto row_count = [&](int i) -> uint32_t {
             if (i < nrow0) {
                 const int z = z0 + i / N0, y = i % N0;
@@ -895,45 +853,23 @@ __device__ void cull_chunk(const CullArgs &a, uint32_t k, uint32_t n, uint64_t *
             }
             return (uint32_t)__popcll(lds[L::P1 + p * n1 + (i - nrow0)]);
         };
-        // list positions: one wave per slice, two rows per lane; the slice
-        // bases come from the slice offsets (L1 rows start after the pair's L0)
-        {
-            const int w = (int)(threadIdx.x >> 6);
-            const int ln = (int)lane_id();
-            const int nsl = nsl0 + (a.lod ? 1 : 0);
-            for (int sl = w; sl < nsl; sl += NW) {
-                const bool l1 = sl == nsl0;
-                const int r0 = l1 ? nrow0 : sl * N0, nr = l1 ? n1 : N0;
-                const uint32_t sbase = l1 ? nl0 : soff[L::SLB0 + z0 + sl] - o0;
-                const int ra = 2 * ln, rb = ra + 1;
-                const uint32_t ca = ra < nr ? row_count(r0 + ra) : 0u;
-                const uint32_t cb = rb < nr ? row_count(r0 + rb) : 0u;
-                uint32_t x = ca + cb;
-#pragma unroll
-                for (int d = 1; d < 64; d <<= 1) {
-                    const uint32_t y = __shfl_up(x, d, 64);
-                    if (ln >= d) x += y;
-                }
-                const uint32_t ex = sbase + x - (ca + cb);
-                if (ra < nr) rowp[r0 + ra] = (uint16_t)ex;
-                if (rb < nr) rowp[r0 + rb] = (uint16_t)(ex + ca);
-            }
-        }
-        __syncthreads();
+        const uint32_t npair = scan_rows<BS>(nrows, row_count, rowp, wsum);
         if (p + 1 < NP) pair_load<B, BS>(a, c, z0 + 2, pr);
+        const uint32_t nl0 = nrow0 < nrows ? (uint32_t)rowp[nrow0] : npair;
+        // expansion: one unit per (row, 16-bit segment)
         const int nunits = nrow0 * NSEG0 + (nrows - nrow0) * NSEG1;
         for (int u = threadIdx.x; u < nunits; u += BS) {
             if (u < nrow0 * NSEG0) {
                 const int i = u / NSEG0, sg = u % NSEG0;
                 const int sidx = i / N0, y = i % N0, z = z0 + sidx;
-                const uint64_t rb = lds[L::V0 + z * N0 + y];
+                const uint64_t rb = row_bits(i);
                 const uint32_t code0 = ((uint32_t)sidx << 14) | ((uint32_t)y << 7);
                 if (sg == NSEG0 - 1) {             // the cell x = R of this row
                     if (z < R && y < R && ((lds[L::XB + z] >> y) & 1ull))
                         list[rowp[i] + __popcll(rb)] = (uint16_t)(code0 | (uint32_t)R);
                 } else {
                     uint32_t pos = rowp[i] + (sg ? (uint32_t)__popcll(rb & ((1ull << (sg * SEGW)) - 1ull)) : 0u);
-                    uint32_t seg = (uint32_t)(rb >> (sg * SEGW)) & ((1u << SEGW) - 1u);
+                    uint32_t seg = (uint32_t)(rb >> (sg * SEGW)) & ((SEGW == 32 ? 0u : (1u << SEGW)) - 1u);
                     while (seg) {
                         const uint32_t x = (uint32_t)(sg * SEGW) + (uint32_t)__builtin_ctz(seg);
                         seg &= seg - 1u;
@@ -943,7 +879,7 @@ __device__ void cull_chunk(const CullArgs &a, uint32_t k, uint32_t n, uint64_t *
             } else {
                 const int v = u - nrow0 * NSEG0, i = nrow0 + v / NSEG1, sg = v % NSEG1;
                 const int y = i - nrow0;
-                const uint64_t rb = lds[L::P1 + p * n1 + y];
+                const uint64_t rb = row_bits(i);
                 uint32_t pos = rowp[i] + (sg ? (uint32_t)__popcll(rb & ((1ull << (sg * 16)) - 1ull)) : 0u);
                 uint32_t seg = (uint32_t)(rb >> (sg * 16)) & 0xFFFFu;
                 while (seg) {
@@ -953,65 +889,18 @@ __device__ void cull_chunk(const CullArgs &a, uint32_t k, uint32_t n, uint64_t *
                 }
             }
         }
-        if (p < nb24) {   // cells of L2..L4 batch p: list positions [p*BT, (p+1)*BT)
-            const uint32_t jb = (uint32_t)p * (uint32_t)BT;
-            for (int i = threadIdx.x; i < NR; i += BS) {
-                uint64_t rb = prow[i];
-                const uint32_t pos0 = rowp24[i], cnt = (uint32_t)__popcll(rb);
-                if (pos0 + cnt <= jb || pos0 >= jb + (uint32_t)BT) continue;
-                const int l = i < G::RB3 - G::RB2 ? 2 : i < G::RB4 - G::RB2 ? 3 : 4;
-                const int nl = G::n(l), rowi = i - (G::row_base(l) - G::RB2);
-                uint32_t pos = pos0;
-                while (rb) {
-                    const uint32_t x = (uint32_t)__builtin_ctzll(rb);
-                    rb &= rb - 1ull;
-                    if (pos >= jb && pos < jb + (uint32_t)BT)
-                        blist[pos - jb] = (uint16_t)(((uint32_t)(l - 2) << 13) | ((uint32_t)rowi * nl + x));
-                    pos++;
-                }
-            }
-        }
+        // Wait for the next pair's prefetch here (vmcnt 0; the encoding leaves
+        // the other counters free), before this pair's stores are issued:
+        // vmcnt retires loads and stores in issue order, so a wait for the
+        // prefetch after the stores would also wait for the stores.
+        __builtin_amdgcn_s_waitcnt(0x0F70);
         __syncthreads();
-        if (p < nb24 && (int)threadIdx.x < BT) {
-            const uint32_t j = (uint32_t)p * (uint32_t)BT + threadIdx.x;
-            if (j < n24) {
-                const uint32_t e = blist[threadIdx.x];
-                const int l = 2 + (int)(e >> 13), nl = G::n(l);
-                const int cell = (int)(e & 0x1FFFu);
-                const int x = cell % nl, y = (cell / nl) % nl, z = cell / (nl * nl);
-                int qx = x, qy = y, qz = z;
-                descend<B, V2>(lds, l, qx, qy, qz);
-                // the level-0 cell's voxel: one load from a valid address
-                // (the chunk itself when the source is uniform or the null chunk)
-                const uint8_t *vb;
-                int vi, u;
-                if (qx == R) { vb = c.nb[0]; vi = (qz * R + qy) * R; u = c.nb_u[0]; }
-                else if (qy == R) { vb = c.nb[1]; vi = (qz * R) * R + qx; u = c.nb_u[1]; }
-                else if (qz == R) { vb = c.nb[2]; vi = qy * R + qx; u = c.nb_u[2]; }
-                else { vb = c.own; vi = (qz * R + qy) * R + qx; u = c.own_u; }
-                const uint8_t *src = u >= 0 ? c.own : vb;
-                const uint32_t raw = src[u >= 0 ? 0 : vi];
-                q_col = u >= 0 ? (uint32_t)u : raw;
-                q_x = c.org[0] + ((uint32_t)x << l);
-                q_y = c.org[1] + ((uint32_t)y << l);
-                q_z = c.org[2] + ((uint32_t)z << l);
-                const uint32_t d = 1u << l;
-                q_wyd = q_y + d;
-                // shadow_sample at +d on all three axes (shadow.h:45-63, u32 index)
-                const uint32_t mi = (q_x + d + q_wyd) + a.sx * (q_z + d);
-                const uint16_t *mp = a.map ? a.map + mi : reinterpret_cast<const uint16_t *>(c.own);
-                q_m0 = mp[0];
-                q_m1 = mp[1];
-                q_fl = 0;
-                if constexpr (V2) q_fl = face_flags<B, V2>(lds, qx, qy, qz);
-                q_j = j;
-            }
-        }
+        const uint32_t o0 = soff[L::SLB0 + z0], o1 = soff[L::SLB1 + p];
         for (uint32_t j = threadIdx.x; j < npair; j += BS) {
             const uint32_t e = list[j];
-            const int x = (int)(e & 0x7Fu);
+            const int x = (int)(e & 0x7Fu), y = (int)((e >> 7) & 0x7Fu);
             if (j < nl0) {
-                const int y = (int)((e >> 7) & 0x7Fu), z = z0 + (int)((e >> 14) & 1u);
+                const int z = z0 + (int)((e >> 14) & 1u);
                 const uint32_t col = staged_colour<B>(stg, x, y, z, z0);
                 const uint32_t wy = c.org[1] + y;
                 const uint32_t sh = staged_shadow<B>(stg, z - z0, x + y, wy);
@@ -1019,12 +908,12 @@ __device__ void cull_chunk(const CullArgs &a, uint32_t k, uint32_t n, uint64_t *
                 if constexpr (V2) fl = face_flags<B, V2>(lds, x, y, z);
                 out[o0 + j] = pack_record<V2>(c.org[0] + x, wy, c.org[2] + z, col, sh, fl);
             } else {
-                const int y = (int)((e >> 7) & 0x3Fu);
-                int qx = x, qy = y, qz = p;
+                const int yy = (int)((e >> 7) & 0x3Fu);
+                int qx = x, qy = yy, qz = p;
                 descend<B, V2>(lds, 1, qx, qy, qz);
                 const uint32_t col = staged_colour<B>(stg, qx, qy, qz, z0);
-                const uint32_t wy = c.org[1] + 2u * y;
-                const uint32_t sh = staged_shadow<B>(stg, 2, 2 * (x + y) + 4, wy + 2u);
+                const uint32_t wy = c.org[1] + 2u * yy;
+                const uint32_t sh = staged_shadow<B>(stg, 2, 2 * (x + yy) + 4, wy + 2u);
                 uint32_t fl = 0;
                 if constexpr (V2) fl = face_flags<B, V2>(lds, qx, qy, qz);
                 out[o1 + (j - nl0)] = pack_record<V2>(c.org[0] + 2u * x, wy, c.org[2] + 2u * p, col,
@@ -1032,9 +921,41 @@ __device__ void cull_chunk(const CullArgs &a, uint32_t k, uint32_t n, uint64_t *
             }
         }
     }
-    if (q_j != 0xFFFFFFFFu) {   // a batch listed during the last pair
-        const uint32_t sh = a.map ? (uint32_t)!(q_m0 <= q_wyd && q_m1 <= q_wyd) : 0u;
-        out[o2 + q_j] = pack_record<V2>(q_x, q_y, q_z, q_col, sh, q_fl);
+    if (!a.lod) return;
+    // levels 2..4: one level at a time, (z, y, x) of each cell in the list
+    for (int l = 2; l <= 4; l++) {
+        const int nl = G::n(l), nrows = nl * nl;
+        const uint64_t *prow = lds + L::P1 + (G::row_base(l) - G::RB1);
+        __syncthreads();
+        auto row_count = [&](int i) -> uint32_t { return (uint32_t)__popcll(prow[i]); };
+        const uint32_t nlev = scan_rows<BS>(nrows, row_count, rowp, wsum);
+        for (int i = threadIdx.x; i < nrows; i += BS) {
+            uint64_t rb = prow[i];
+            uint32_t pos = rowp[i];
+            const uint32_t zy = ((uint32_t)(i / nl) << 10) | ((uint32_t)(i % nl) << 5);
+            while (rb) {
+                const uint32_t x = (uint32_t)__builtin_ctzll(rb);
+                rb &= rb - 1ull;
+                list[pos++] = (uint16_t)(zy | x);
+            }
+        }
+        __syncthreads();
+        const uint32_t ol = soff[l == 2 ? L::SLB2 : l == 3 ? L::SLB3 : L::SLB4];
+        const uint32_t d = 1u << l;
+        for (uint32_t j = threadIdx.x; j < nlev; j += BS) {
+            const uint32_t e = list[j];
+            const int x = (int)(e & 31u), y = (int)((e >> 5) & 31u), z = (int)(e >> 10);
+            int qx = x, qy = y, qz = z;
+            descend<B, V2>(lds, l, qx, qy, qz);
+            const uint32_t col = l0_colour<B>(c, qx, qy, qz);
+            const uint32_t wx = c.org[0] + ((uint32_t)x << l);
+            const uint32_t wy = c.org[1] + ((uint32_t)y << l);
+            const uint32_t wz = c.org[2] + ((uint32_t)z << l);
+            const uint32_t sh = shadow_bit(a.map, a.sx, wx + d, wy + d, wz + d);
+            uint32_t fl = 0;
+            if constexpr (V2) fl = face_flags<B, V2>(lds, qx, qy, qz);
+            out[ol + j] = pack_record<V2>(wx, wy, wz, col, sh, fl);
+        }
     }
 }
 

@@ -68,6 +68,23 @@ class ResidentWorld:
         return self.voxels[c * self.R3:(c + 1) * self.R3]
 
 
+def heaviest_first(gpu: GPU, w: abi.WorldDesc, voxels, ids, smap, uniform, lod: bool = True):
+    """Reorder a chunk list by record count, heaviest first, and size it.
+
+    The batched remesh hands chunks to workgroups in list order and writes
+    its output in list order, so the order is the caller's schedule: with the
+    heaviest chunks first, the chunks still running when the list runs dry
+    are the light ones.  Returns (ids, offsets, counts) for the new order,
+    offsets and counts from a counting pass over it."""
+    _, offs, counts = gpu.cull_mesh(w, voxels, ids, shadow_map=smap, uniform=uniform, lod=lod,
+                                    count_only=True)
+    order = torch.argsort(counts.sum(1), descending=True, stable=True)
+    ids = ids[order].contiguous()
+    _, offs, counts = gpu.cull_mesh(w, voxels, ids, shadow_map=smap, uniform=uniform, lod=lod,
+                                    count_only=True)
+    return ids, offs, counts
+
+
 class Frame:
     """One frame of chunk management with every buffer sized up front: the
     remesh of `dirty` is captured in a hipGraph and the RLE of `idle` reuses
@@ -75,11 +92,15 @@ class Frame:
 
     def __init__(self, world: ResidentWorld, dirty, idle, lod: bool = True):
         gpu, w = world.gpu, world.w
+        if dirty.numel():
+            dirty, self.offsets, self.counts = heaviest_first(gpu, w, world.voxels, dirty,
+                                                              world.smap, world.uniform, lod)
+        else:
+            _, self.offsets, self.counts = gpu.cull_mesh(w, world.voxels, dirty,
+                                                         shadow_map=world.smap,
+                                                         uniform=world.uniform, lod=lod,
+                                                         count_only=True)
         self.world, self.dirty, self.idle = world, dirty, idle
-        _, self.offsets, self.counts = gpu.cull_mesh(w, world.voxels, dirty,
-                                                     shadow_map=world.smap,
-                                                     uniform=world.uniform, lod=lod,
-                                                     count_only=True)
         _, self.word_offsets, self.idle_uniform = gpu.rle_encode(w, world.voxels, idle,
                                                                  count_only=True)
         torch.cuda.synchronize()
@@ -172,12 +193,12 @@ def rle_bytes(n_chunks: int, words: int, R3: int = 262144) -> int:
 
 
 def full_remesh(world: ResidentWorld, steps: int = 3, warmup: int = 1, log=None):
-    """Remesh every chunk of the world (L0-L4) as one graph launch per step.
-    Returns timings; the record buffer is freed before returning."""
+    """Remesh every chunk of the world (L0-L4) as one graph launch per step,
+    the list ordered heaviest first.  Returns timings; the record buffer is
+    freed before returning."""
     gpu, w = world.gpu, world.w
     ids = torch.arange(w.chunks, dtype=torch.int32, device=world.voxels.device)
-    _, offs, counts = gpu.cull_mesh(w, world.voxels, ids, shadow_map=world.smap,
-                                    uniform=world.uniform, count_only=True)
+    ids, offs, counts = heaviest_first(gpu, w, world.voxels, ids, world.smap, world.uniform)
     torch.cuda.synchronize()
     records = int(offs[-1].item()) // 4
     if log:
