@@ -2,11 +2,12 @@
 
     python -m voxplat_amd.remesh_run [--world 2k|16k] [--launches K] [--copy-gib G]
 
-Generates the world on the GPU, sizes the output with a counting pass, then
-issues K direct (not graph-captured) launches of vp_gpu_cull_mesh over every
-chunk, so that a kernel tracer or counter collector sees one dispatch per
-launch.  --copy-gib G first runs one device-to-device copy of G GiB: a kernel
-of known traffic (G GiB read, G GiB written) against which HBM byte counters
+Generates the world on the GPU, orders the chunk list heaviest first and
+sizes the output with counting passes, then issues K direct (not
+graph-captured) launches of vp_gpu_cull_mesh over every chunk, so that a
+kernel tracer or counter collector sees one dispatch per launch.
+--copy-gib G first runs one device-to-device copy of G GiB: a kernel of
+known traffic (G GiB read, G GiB written) against which HBM byte counters
 are scaled.
 """
 from __future__ import annotations
@@ -20,6 +21,7 @@ import torch
 from . import abi
 from . import workload as W
 from ._lib import GPU
+from .world_frame import heaviest_first
 
 
 def main(argv=None) -> int:
@@ -27,6 +29,7 @@ def main(argv=None) -> int:
     ap.add_argument("--world", choices=("2k", "16k"), default="2k")
     ap.add_argument("--launches", type=int, default=3)
     ap.add_argument("--copy-gib", type=float, default=0.0)
+    ap.add_argument("--autotune", action="store_true", help="pick the launch shape first")
     a = ap.parse_args(argv)
     gpu = GPU(0)
     if a.copy_gib:
@@ -46,10 +49,14 @@ def main(argv=None) -> int:
     smap = torch.empty(abi.shadow_padded_len(sx, sz), dtype=torch.int16, device="cuda")
     gpu.shadow_build(w, vox, smap)
     ids = torch.arange(w.chunks, dtype=torch.int32, device="cuda")
-    _, offs, counts = gpu.cull_mesh(w, vox, ids, shadow_map=smap, uniform=uni, count_only=True)
+    ids, offs, counts = heaviest_first(gpu, w, vox, ids, smap, uni)
     torch.cuda.synchronize()
     cap = int(offs[-1].item()) // 4
     splats = torch.empty((max(cap, 1), 4), dtype=torch.int16, device="cuda")
+    if a.autotune:   # the launch shape the benchmark would pick
+        b, wg, _ = gpu.autotune_remesh(w, vox, ids, smap, splats, cap, offs, counts, uniform=uni,
+                                       reps=3)
+        print(f"launch shape: {b} threads x {wg} per CU", file=sys.stderr, flush=True)
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     ms = []
     for _ in range(a.launches):
