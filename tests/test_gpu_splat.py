@@ -15,14 +15,15 @@ DEPTH_RTOL = 1e-5     # fp32 depth tolerance (relative)
 RGB_TOL = 1e-5
 
 
-def _compare(gpu, splats, cam, lod=None, lod_log2=0, voxel_rgb=None):
+def _compare(gpu, splats, cam, lod=None, lod_log2=0, voxel_rgb=None, blocks=False):
     import torch
 
     sd = dev(np.ascontiguousarray(splats, np.int16))
     ld = None if lod is None else dev(np.ascontiguousarray(lod, np.uint8))
     rd = None if voxel_rgb is None else dev(np.ascontiguousarray(voxel_rgb, np.float32))
+    bounds = gpu.splat_bounds(sd, lod=ld, lod_log2=lod_log2) if blocks else None
     depth, ids, rgb, wt = gpu.splat(sd, cam, lod=ld, lod_log2=lod_log2, voxel_rgb=rd, rgb=True,
-                                    weight=True)
+                                    weight=True, bounds=bounds)
     torch.cuda.synchronize()
     ed, ei, er, ew = O.splat_render(splats, cam, lod=lod, lod_log2=lod_log2,
                                     voxel_rgb=voxel_rgb, rgb=True, weight=True)
@@ -105,3 +106,40 @@ def test_backward_matches_oracle(gpu, terrain):
     gpu.splat_backward(n, cam, dev(ids), dev(ew), dev(G), grad)
     torch.cuda.synchronize()
     assert np.allclose(host(grad), eg, rtol=1e-5, atol=1e-5)
+
+
+def test_block_bounds(gpu, terrain):
+    w, recs, lod, l0, top = terrain
+    b = host(gpu.splat_bounds(dev(recs), lod=dev(lod)))
+    n = len(recs)
+    half = 0.5 * (1 << lod.astype(np.int64))
+    cen = recs[:, :3].astype(np.float64) + half[:, None]
+    nb = (n + 255) // 256
+    assert b.shape == (nb, 8)
+    for k in (0, 1, nb // 2, nb - 1):
+        s = slice(256 * k, min(n, 256 * (k + 1)))
+        assert np.array_equal(b[k, :3], cen[s].min(0).astype(np.float32))
+        assert b[k, 3] == np.float32(half[s].max())
+        assert np.array_equal(b[k, 4:7], cen[s].max(0).astype(np.float32))
+
+
+def test_block_path_matches_oracle(gpu, terrain):
+    """Culled, near-to-far block raster: the same ids and depths as the oracle,
+    level 0 and mixed LOD, from above the terrain and from inside the world."""
+    w, recs, lod, l0, top = terrain
+    poses = W.splat_poses(top, n=3, seed=57, lo=40.0, hi=216.0)
+    poses.append(((128.0, 60.0, 128.0), 30.0, -5.0))      # among the hills, looking out
+    for loc, yaw, pitch in poses:
+        cam = W.make_camera(loc, yaw, pitch, 320, 180)
+        _compare(gpu, l0, cam, blocks=True)
+    loc, yaw, pitch = poses[0]
+    _compare(gpu, recs, W.make_camera(loc, yaw, pitch, 256, 160), lod=lod, blocks=True)
+
+
+def test_block_path_everything_culled_or_empty(gpu):
+    import torch
+
+    splats = np.array([[x, 0, 0, 5] for x in range(600)], np.int16)
+    cam = W.make_camera((300.0, 10.0, 300.0), 180.0, 0.0, 64, 48)   # looking away from the row
+    ids, depth = _compare(gpu, splats, cam, blocks=True)
+    assert (ids == 0xFFFFFFFF).all() and (depth == 1.0).all()

@@ -196,8 +196,21 @@ class GPU:
         return voxels
 
     # ---- splat -----------------------------------------------------------------
+    def splat_bounds(self, splats, lod=None, lod_log2: int = 0, stream=None):
+        """Per-block bounds (float32 [ceil(n/256), 8]) for splat(..., bounds=...)."""
+        torch = self.torch
+        n = int(splats.shape[0])
+        nb = (n + abi.VP_SPLAT_BLOCK - 1) // abi.VP_SPLAT_BLOCK
+        b = torch.empty((max(nb, 1), 8), dtype=torch.float32, device=splats.device)
+        _check(self.lib.vp_gpu_splat_bounds(self.ctx, C.c_void_p(splats.data_ptr()),
+                                            C.c_void_p(_ptr(lod)), lod_log2, n,
+                                            C.c_void_p(b.data_ptr()),
+                                            C.c_void_p(self._stream(stream))),
+               "vp_gpu_splat_bounds")
+        return b
+
     def splat(self, splats, cam: abi.Camera, lod=None, lod_log2: int = 0, voxel_rgb=None,
-              rgb: bool = False, weight: bool = False, out=None, stream=None):
+              rgb: bool = False, weight: bool = False, out=None, stream=None, bounds=None):
         torch = self.torch
         W, H = cam.width, cam.height
         dev = splats.device
@@ -212,10 +225,18 @@ class GPU:
         fr = abi.Frame()
         fr.d_depth, fr.d_id, fr.d_rgb, fr.d_weight = (depth.data_ptr(), ids.data_ptr(),
                                                       _ptr(col), _ptr(wt))
-        _check(self.lib.vp_gpu_splat(self.ctx, C.c_void_p(splats.data_ptr()),
-                                     C.c_void_p(_ptr(lod)), lod_log2, n, C.byref(cam),
-                                     C.c_void_p(_ptr(voxel_rgb)), C.byref(fr),
-                                     C.c_void_p(self._stream(stream))), "vp_gpu_splat")
+        if bounds is None:
+            _check(self.lib.vp_gpu_splat(self.ctx, C.c_void_p(splats.data_ptr()),
+                                         C.c_void_p(_ptr(lod)), lod_log2, n, C.byref(cam),
+                                         C.c_void_p(_ptr(voxel_rgb)), C.byref(fr),
+                                         C.c_void_p(self._stream(stream))), "vp_gpu_splat")
+        else:
+            _check(self.lib.vp_gpu_splat_blocks(self.ctx, C.c_void_p(splats.data_ptr()),
+                                                C.c_void_p(_ptr(lod)), lod_log2, n,
+                                                C.c_void_p(bounds.data_ptr()), C.byref(cam),
+                                                C.c_void_p(_ptr(voxel_rgb)), C.byref(fr),
+                                                C.c_void_p(self._stream(stream))),
+                   "vp_gpu_splat_blocks")
         return depth, ids, col, wt
 
     def splat_backward(self, n: int, cam: abi.Camera, ids, weight, dL_drgb, grad, stream=None):

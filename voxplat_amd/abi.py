@@ -98,7 +98,13 @@ PROTOTYPES = {
                                C.POINTER(Camera), C.c_void_p, C.POINTER(Frame), C.c_void_p]),
     "vp_gpu_splat_backward": (C.c_int, [C.c_void_p, C.c_uint64, C.POINTER(Camera), C.c_void_p,
                                         C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "vp_gpu_splat_bounds": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint64,
+                                      C.c_void_p, C.c_void_p]),
+    "vp_gpu_splat_blocks": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint64,
+                                      C.c_void_p, C.POINTER(Camera), C.c_void_p, C.POINTER(Frame),
+                                      C.c_void_p]),
 }
+VP_SPLAT_BLOCK = 256
 
 
 # ---- include/vp_workload.h ---------------------------------------------------

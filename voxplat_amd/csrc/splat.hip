@@ -49,6 +49,7 @@ struct VS {
     float vx, vy, vz, half;
     float lox, hix, loy, hiy;
     int x0, x1, y0, y1;
+    uint64_t lb;   // (lower bound of the splat's depth t0/far) << 32: no fragment key is below it
 };
 
 __device__ __forceinline__ bool vertex_stage(const int16_t *rec, float lodf, const SplatCam &c,
@@ -79,14 +80,27 @@ __device__ __forceinline__ bool vertex_stage(const int16_t *rec, float lodf, con
     const float hs = 0.5f * ps;
     o.lox = wx - hs; o.hix = wx + hs;
     o.loy = wy - hs; o.hiy = wy + hs;
-    const int x0 = (int)floorf(o.lox) - 1, x1 = (int)ceilf(o.hix) + 1;
-    const int y0 = (int)floorf(o.loy) - 1, y1 = (int)ceilf(o.hiy) + 1;
+    // pixel p is covered iff lo <= p + 0.5 < hi (tested exactly per pixel);
+    // every such p lies in [floor(lo), ceil(hi) - 1]
+    const int x0 = (int)floorf(o.lox), x1 = (int)ceilf(o.hix) - 1;
+    const int y0 = (int)floorf(o.loy), y1 = (int)ceilf(o.hiy) - 1;
     o.x0 = x0 < 0 ? 0 : x0;
     o.y0 = y0 < 0 ? 0 : y0;
     o.x1 = x1 > (int)c.W - 1 ? (int)c.W - 1 : x1;
     o.y1 = y1 > (int)c.H - 1 ? (int)c.H - 1 : y1;
     o.vx = v[0]; o.vy = v[1]; o.vz = v[2];
     o.half = half;
+    // Distance from the eye (the origin of the translated frame) to the box
+    // v +- half bounds every ray's entry time t0 from below; shrunk by 2^-10,
+    // far more than the rounding of the slab test, it stays below the computed
+    // t0, and t0/far keeps the order after rounding.
+    const float ex = gmax(fabsf(v[0]) - half, 0.0f), ey = gmax(fabsf(v[1]) - half, 0.0f);
+    const float ez = gmax(fabsf(v[2]) - half, 0.0f);
+    float dd = ex * ex;
+    dd = dd + ey * ey;
+    dd = dd + ez * ez;
+    const float lbd = (sqrtf(dd) * 0.9990234375f) / c.far_plane;
+    o.lb = (uint64_t)__float_as_uint(lbd) << 32;
     return true;
 }
 
@@ -135,9 +149,11 @@ __global__ void k_clear(uint64_t *frame, uint64_t np)
 
 __device__ __forceinline__ void fragment(const SplatCam &c, const float4 *rays, uint64_t *frame,
                                          float vx, float vy, float vz, float half, uint32_t id,
-                                         int px, int py)
+                                         uint64_t lb, int px, int py)
 {
     const size_t p = (size_t)py * c.W + (uint32_t)px;
+    const uint64_t cur = frame[p];
+    if (lb > cur) return;          // every key of this splat is above the stored one
     const float4 inv = rays[p];
     float t0, t1;
     aabb(vx, vy, vz, half, inv.x, inv.y, inv.z, t0, t1);
@@ -146,18 +162,19 @@ __device__ __forceinline__ void fragment(const SplatCam &c, const float4 *rays, 
     if (!(d < 1.0f)) return;
     if (d <= 0.0f) d = 0.0f;
     const uint64_t key = ((uint64_t)__float_as_uint(d) << 32) | id;
-    if (key < frame[p]) atomicMin(reinterpret_cast<unsigned long long *>(frame + p), key);
+    if (key < cur) atomicMin(reinterpret_cast<unsigned long long *>(frame + p), key);
 }
 
 constexpr int kSmallArea = 16;
 
-// One thread per splat; sprites larger than kSmallArea pixels are shared by
-// the whole wave, one at a time, so a few near splats do not stall a wave.
-__global__ void __launch_bounds__(256) k_raster(const int16_t *splats, const uint8_t *lod,
-                                                uint32_t lod_log2, uint64_t n, SplatCam c,
-                                                const float4 *rays, uint64_t *frame)
+// Splat i (i >= n: none) of one lane; every lane of the wave calls it.
+// Sprites larger than kSmallArea pixels are shared by the whole wave, one at
+// a time, so a few near splats do not stall a wave.
+__device__ __forceinline__ void raster_splat(const int16_t *splats, const uint8_t *lod,
+                                             uint32_t lod_log2, uint64_t n, uint64_t i,
+                                             const SplatCam &c, const float4 *rays,
+                                             uint64_t *frame)
 {
-    const uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
     VS vs;
     bool ok = false;
     if (i < n) {
@@ -174,7 +191,7 @@ __global__ void __launch_bounds__(256) k_raster(const int16_t *splats, const uin
             for (int px = vs.x0; px <= vs.x1; px++) {
                 const float cx = (float)px + 0.5f;
                 if (!(vs.lox <= cx && cx < vs.hix)) continue;
-                fragment(c, rays, frame, vs.vx, vs.vy, vs.vz, vs.half, id, px, py);
+                fragment(c, rays, frame, vs.vx, vs.vy, vs.vz, vs.half, id, vs.lb, px, py);
             }
         }
     }
@@ -191,13 +208,164 @@ __global__ void __launch_bounds__(256) k_raster(const int16_t *splats, const uin
         const int y0 = __shfl(vs.y0, src, 64);
         const int a = __shfl(area, src, 64);
         const uint32_t sid = __shfl(id, src, 64);
+        const uint64_t lb = ((uint64_t)(uint32_t)__shfl((int)(uint32_t)(vs.lb >> 32), src, 64)) << 32;
+        // lane k walks pixels k, k+64, ... of the box row by row; the column
+        // and row advance by 64 % w and 64 / w per step (one division per sprite)
         const int w = x1 - x0 + 1;
+        const int dq = 64 / w, dr = 64 - dq * w;
+        int col = lane % w, row = lane / w;
         for (int k = lane; k < a; k += 64) {
-            const int px = x0 + k % w, py = y0 + k / w;
+            const int px = x0 + col, py = y0 + row;
+            col += dr;
+            row += dq;
+            if (col >= w) { col -= w; row += 1; }
             const float cx = (float)px + 0.5f, cy = (float)py + 0.5f;
             if (!(lox <= cx && cx < hix && loy <= cy && cy < hiy)) continue;
-            fragment(c, rays, frame, vx, vy, vz, half, sid, px, py);
+            fragment(c, rays, frame, vx, vy, vz, half, sid, lb, px, py);
         }
+    }
+}
+
+// One thread per splat, in list order.
+__global__ void __launch_bounds__(256) k_raster(const int16_t *splats, const uint8_t *lod,
+                                                uint32_t lod_log2, uint64_t n, SplatCam c,
+                                                const float4 *rays, uint64_t *frame)
+{
+    const uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    raster_splat(splats, lod, lod_log2, n, i, c, rays, frame);
+}
+
+// ---- block culling and near-to-far order -------------------------------------
+// Block b is splats [256b, 256b+256); its bounds are the box of the splat
+// centres and their largest half-size.  Blocks wholly outside the view volume
+// are dropped; the rest are bucketed by distance and rasterised nearest bucket
+// first, so the early depth rejection in fragment() sees the occluders first.
+constexpr int kBlock = VP_SPLAT_BLOCK;
+constexpr int kBins = 256;
+
+__global__ void __launch_bounds__(256) k_bounds(const int16_t *splats, const uint8_t *lod,
+                                                uint32_t lod_log2, uint64_t n, float *bounds)
+{
+    __shared__ float red[8][7];
+    const uint64_t b = blockIdx.x, i = b * kBlock + threadIdx.x;
+    float v[7];   // min x y z, max half, -max x y z (all reduced with min)
+    if (i < n) {
+        const float half = 0.5f * (float)(1u << (lod ? lod[i] : lod_log2));
+        const int16_t *r = splats + 4 * i;
+        const float cx = (float)r[0] + half, cy = (float)r[1] + half, cz = (float)r[2] + half;
+        v[0] = cx; v[1] = cy; v[2] = cz; v[3] = -half; v[4] = -cx; v[5] = -cy; v[6] = -cz;
+    } else {
+        for (int k = 0; k < 7; k++) v[k] = INFINITY;
+    }
+    for (int k = 0; k < 7; k++)
+        for (int d = 32; d >= 1; d >>= 1) v[k] = fminf(v[k], __shfl_xor(v[k], d, 64));
+    const int w = (int)(threadIdx.x >> 6);
+    if ((threadIdx.x & 63u) == 0)
+        for (int k = 0; k < 7; k++) red[w][k] = v[k];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        float m[7];
+        for (int k = 0; k < 7; k++) {
+            m[k] = red[0][k];
+            for (int j = 1; j < (int)(blockDim.x >> 6); j++) m[k] = fminf(m[k], red[j][k]);
+        }
+        float *o = bounds + 8 * b;
+        o[0] = m[0]; o[1] = m[1]; o[2] = m[2]; o[3] = -m[3];
+        o[4] = -m[4]; o[5] = -m[5]; o[6] = -m[6]; o[7] = 0.0f;
+    }
+}
+
+struct CullPlanes {
+    float p[6][4];   // inside: p.x*cx + p.y*cy + p.z*cz + p.w >= 0 for a splat centre
+    float eye[3];
+    float far_plane;
+};
+
+__global__ void k_blk_reset(uint32_t *cnt)
+{
+    for (int i = threadIdx.x; i < 2 * kBins + 2; i += blockDim.x) cnt[i] = 0u;
+}
+
+__global__ void k_blk_cull(const float *bounds, uint64_t nb, CullPlanes cp, uint32_t *bin,
+                           uint32_t *cnt)
+{
+    for (uint64_t b = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; b < nb;
+         b += (uint64_t)gridDim.x * blockDim.x) {
+        const float *o = bounds + 8 * b;
+        const float lo[3] = {o[0], o[1], o[2]}, hi[3] = {o[4], o[5], o[6]};
+        bool out = !(lo[0] <= hi[0]);   // empty block
+        for (int k = 0; k < 6 && !out; k++) {
+            // the largest value of the plane function over the box of centres
+            float f = cp.p[k][3], mag = fabsf(cp.p[k][3]);
+            for (int a = 0; a < 3; a++) {
+                const float c = cp.p[k][a];
+                f += c * (c > 0.0f ? hi[a] : lo[a]);
+                mag += fabsf(c) * fmaxf(fabsf(lo[a]), fabsf(hi[a]));
+            }
+            // the rounding of the vertex stage's clip test is far below 1e-3 of mag
+            out = f < -1e-3f * mag;
+        }
+        if (out) {
+            bin[b] = 0xFFFFFFFFu;
+            continue;
+        }
+        float dd = 0.0f;
+        for (int a = 0; a < 3; a++) {
+            const float e = fmaxf(fmaxf(lo[a] - o[3] - cp.eye[a], cp.eye[a] - hi[a] - o[3]), 0.0f);
+            dd += e * e;
+        }
+        const float t = sqrtf(sqrtf(dd) / cp.far_plane);
+        const uint32_t k = (uint32_t)fminf(t * (float)kBins, (float)(kBins - 1));
+        bin[b] = k;
+        atomicAdd(cnt + k, 1u);
+    }
+}
+
+// exclusive scan of the bin counts into the bin cursors; cnt[2*kBins] = total
+__global__ void __launch_bounds__(kBins) k_blk_scan(uint32_t *cnt)
+{
+    __shared__ uint32_t sm[kBins];
+    const uint32_t t = threadIdx.x;
+    sm[t] = cnt[t];
+    __syncthreads();
+    for (uint32_t d = 1; d < kBins; d <<= 1) {
+        const uint32_t v = t >= d ? sm[t - d] : 0u;
+        __syncthreads();
+        sm[t] += v;
+        __syncthreads();
+    }
+    cnt[kBins + t] = sm[t] - cnt[t];
+    if (t == kBins - 1) cnt[2 * kBins] = sm[t];
+}
+
+__global__ void k_blk_scatter(const uint32_t *bin, uint64_t nb, uint32_t *cnt, uint32_t *order)
+{
+    for (uint64_t b = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; b < nb;
+         b += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t k = bin[b];
+        if (k == 0xFFFFFFFFu) continue;
+        order[atomicAdd(cnt + kBins + k, 1u)] = (uint32_t)b;
+    }
+}
+
+// Persistent: workgroups take the ordered blocks through a ticket, so blocks
+// start nearest first.
+__global__ void __launch_bounds__(kBlock) k_raster_blocks(const int16_t *splats,
+                                                           const uint8_t *lod, uint32_t lod_log2,
+                                                           uint64_t n, SplatCam c,
+                                                           const float4 *rays, uint64_t *frame,
+                                                           const uint32_t *order, uint32_t *cnt)
+{
+    __shared__ uint32_t s_t;
+    const uint32_t total = cnt[2 * kBins];
+    for (;;) {
+        if (threadIdx.x == 0) s_t = atomicAdd(cnt + 2 * kBins + 1, 1u);
+        __syncthreads();
+        const uint32_t t = s_t;
+        __syncthreads();
+        if (t >= total) break;
+        const uint64_t i = (uint64_t)order[t] * kBlock + threadIdx.x;
+        raster_splat(splats, lod, lod_log2, n, i, c, rays, frame);
     }
 }
 
@@ -302,9 +470,32 @@ static SplatCam to_cam(const vp_camera *cam)
 
 using namespace vp;
 
-extern "C" int vp_gpu_splat(vp_ctx *ctx, const int16_t *d_splats, const uint8_t *d_lod,
-                            uint32_t lod_log2, uint64_t n, const vp_camera *cam,
-                            const float *d_voxel_rgb, const vp_frame *out, void *stream)
+static CullPlanes cull_planes(const vp_camera *cam)
+{
+    // M = proj_view_rot * view_translation (column-major), in double: clip = M (c, 1)
+    double M[4][4];
+    for (int r = 0; r < 4; r++)
+        for (int k = 0; k < 4; k++) {
+            double acc = 0.0;
+            for (int j = 0; j < 4; j++)
+                acc += (double)cam->proj_view_rot[j * 4 + r] * (double)cam->view_translation[k * 4 + j];
+            M[r][k] = acc;
+        }
+    CullPlanes cp;
+    // -w <= x, x <= w, -w <= y, y <= w, -w <= z, z <= w
+    for (int a = 0; a < 3; a++)
+        for (int sgn = 0; sgn < 2; sgn++)
+            for (int k = 0; k < 4; k++)
+                cp.p[2 * a + sgn][k] = (float)(M[3][k] + (sgn ? -M[a][k] : M[a][k]));
+    for (int a = 0; a < 3; a++) cp.eye[a] = -cam->view_translation[12 + a];
+    cp.far_plane = cam->far_plane;
+    return cp;
+}
+
+static int splat_frame(vp_ctx *ctx, const int16_t *d_splats, const uint8_t *d_lod,
+                       uint32_t lod_log2, uint64_t n, const float *d_bounds,
+                       const vp_camera *cam, const float *d_voxel_rgb, const vp_frame *out,
+                       void *stream)
 {
     if (!ctx || !cam || !out || !out->d_depth || !out->d_id) return VP_ERR_ARG;
     if (n && !d_splats) return VP_ERR_ARG;
@@ -313,23 +504,42 @@ extern "C" int vp_gpu_splat(vp_ctx *ctx, const int16_t *d_splats, const uint8_t 
     if (lod_log2 > 8) return VP_ERR_ARG;
     if (n > 0xFFFFFFFFull) return VP_ERR_UNSUPPORTED;   // ids are 32-bit
     const uint64_t np = (uint64_t)cam->width * cam->height;
+    const uint64_t nb = (n + kBlock - 1) / kBlock;
     hipStream_t st = as_stream(stream);
     uint64_t *frame;
     float4 *rays;
+    uint32_t *blk = nullptr;
+    int ncu;
     {
         std::lock_guard<std::mutex> g(ctx->mu);
         int rc = ensure_frame(ctx, np);
         if (rc) return rc;
+        if (d_bounds && nb) {
+            rc = ensure_blocks(ctx, nb);
+            if (rc) return rc;
+            blk = ctx->d_blk;
+        }
         frame = ctx->d_frame;
         rays = ctx->d_rays;
+        ncu = ctx->num_cus;
     }
     const SplatCam c = to_cam(cam);
     const dim3 pg((cam->width + 255) / 256, cam->height);
     hipLaunchKernelGGL(k_rays, pg, dim3(256), 0, st, c, rays);
     hipLaunchKernelGGL(k_clear, dim3(1024), dim3(256), 0, st, frame, np);
-    if (n) {
-        const uint64_t blocks = (n + 255) / 256;
-        hipLaunchKernelGGL(k_raster, dim3((uint32_t)blocks), dim3(256), 0, st, d_splats, d_lod,
+    if (n && blk) {
+        // blk: [bin per block | order | counts (2*kBins+2)]
+        uint32_t *bin = blk, *order = blk + nb, *cnt = blk + 2 * nb;
+        const uint32_t g = (uint32_t)((nb + 255) / 256 < 4096 ? (nb + 255) / 256 : 4096);
+        hipLaunchKernelGGL(k_blk_reset, dim3(1), dim3(256), 0, st, cnt);
+        hipLaunchKernelGGL(k_blk_cull, dim3(g), dim3(256), 0, st, d_bounds, nb, cull_planes(cam),
+                           bin, cnt);
+        hipLaunchKernelGGL(k_blk_scan, dim3(1), dim3(kBins), 0, st, cnt);
+        hipLaunchKernelGGL(k_blk_scatter, dim3(g), dim3(256), 0, st, bin, nb, cnt, order);
+        hipLaunchKernelGGL(k_raster_blocks, dim3((uint32_t)ncu * 8u), dim3(kBlock), 0, st, d_splats,
+                           d_lod, lod_log2, n, c, rays, frame, order, cnt);
+    } else if (n) {
+        hipLaunchKernelGGL(k_raster, dim3((uint32_t)nb), dim3(256), 0, st, d_splats, d_lod,
                            lod_log2, n, c, rays, frame);
     }
     hipLaunchKernelGGL(k_resolve, pg, dim3(256), 0, st, d_splats, d_lod, lod_log2, n, c,
@@ -337,6 +547,35 @@ extern "C" int vp_gpu_splat(vp_ctx *ctx, const int16_t *d_splats, const uint8_t 
                        out->d_weight);
     VP_HIP_TRY(hipGetLastError());
     return VP_OK;
+}
+
+extern "C" int vp_gpu_splat(vp_ctx *ctx, const int16_t *d_splats, const uint8_t *d_lod,
+                            uint32_t lod_log2, uint64_t n, const vp_camera *cam,
+                            const float *d_voxel_rgb, const vp_frame *out, void *stream)
+{
+    return splat_frame(ctx, d_splats, d_lod, lod_log2, n, nullptr, cam, d_voxel_rgb, out, stream);
+}
+
+extern "C" int vp_gpu_splat_bounds(vp_ctx *ctx, const int16_t *d_splats, const uint8_t *d_lod,
+                                   uint32_t lod_log2, uint64_t n, float *d_bounds, void *stream)
+{
+    if (!ctx || (n && (!d_splats || !d_bounds))) return VP_ERR_ARG;
+    if (lod_log2 > 8) return VP_ERR_ARG;
+    if (n > 0xFFFFFFFFull) return VP_ERR_UNSUPPORTED;
+    if (n == 0) return VP_OK;
+    const uint64_t nb = (n + kBlock - 1) / kBlock;
+    hipLaunchKernelGGL(k_bounds, dim3((uint32_t)nb), dim3(kBlock), 0, as_stream(stream), d_splats,
+                       d_lod, lod_log2, n, d_bounds);
+    VP_HIP_TRY(hipGetLastError());
+    return VP_OK;
+}
+
+extern "C" int vp_gpu_splat_blocks(vp_ctx *ctx, const int16_t *d_splats, const uint8_t *d_lod,
+                                   uint32_t lod_log2, uint64_t n, const float *d_bounds,
+                                   const vp_camera *cam, const float *d_voxel_rgb,
+                                   const vp_frame *out, void *stream)
+{
+    return splat_frame(ctx, d_splats, d_lod, lod_log2, n, d_bounds, cam, d_voxel_rgb, out, stream);
 }
 
 extern "C" int vp_gpu_splat_backward(vp_ctx *ctx, uint64_t n, const vp_camera *cam,

@@ -42,6 +42,22 @@ int ensure_frame(vp_ctx *ctx, uint64_t pixels)
     return VP_OK;
 }
 
+int ensure_blocks(vp_ctx *ctx, uint64_t blocks)
+{
+    if (blocks <= ctx->blk_cap) return VP_OK;
+    uint32_t *p = nullptr;
+    // a bin and an order slot per block, then 2 * 256 bin counters, total, ticket
+    if (hipMalloc(&p, (2 * blocks + 2 * 256 + 2) * sizeof(uint32_t)) != hipSuccess)
+        return VP_ERR_NOMEM;
+    if (ctx->d_blk) {
+        if (hipDeviceSynchronize() != hipSuccess) return VP_ERR_HIP;
+        (void)hipFree(ctx->d_blk);
+    }
+    ctx->d_blk = p;
+    ctx->blk_cap = blocks;
+    return VP_OK;
+}
+
 }  // namespace vp
 
 extern "C" int vp_ctx_create(int device, vp_ctx **out)
@@ -83,6 +99,7 @@ extern "C" int vp_ctx_destroy(vp_ctx *ctx)
     if (ctx->d_state) (void)hipFree(ctx->d_state);
     if (ctx->d_frame) (void)hipFree(ctx->d_frame);
     if (ctx->d_rays) (void)hipFree(ctx->d_rays);
+    if (ctx->d_blk) (void)hipFree(ctx->d_blk);
     delete ctx;
     return VP_OK;
 }

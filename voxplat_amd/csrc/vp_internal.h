@@ -27,6 +27,8 @@ struct vp_ctx {
     uint64_t *d_frame = nullptr;       // packed depth|id per pixel for splat
     uint64_t  frame_cap = 0;
     float4   *d_rays = nullptr;        // per-pixel 1/ray direction for splat
+    uint32_t *d_blk = nullptr;         // splat block culling: bins, order, counters
+    uint64_t  blk_cap = 0;             // blocks d_blk is sized for
     uint64_t  rays_cap = 0;
     // tuned launch shape of the persistent remesh kernel
     int cull_block = 512;              // threads per workgroup
@@ -37,6 +39,7 @@ namespace vp {
 
 int ensure_state(vp_ctx *ctx, uint64_t words);
 int ensure_frame(vp_ctx *ctx, uint64_t pixels);
+int ensure_blocks(vp_ctx *ctx, uint64_t blocks);
 
 inline hipStream_t as_stream(void *s) { return reinterpret_cast<hipStream_t>(s); }
 
