@@ -210,10 +210,14 @@ __device__ __forceinline__ void raster_splat(const int16_t *splats, const uint8_
         const uint32_t sid = __shfl(id, src, 64);
         const uint64_t lb = ((uint64_t)(uint32_t)__shfl((int)(uint32_t)(vs.lb >> 32), src, 64)) << 32;
         // lane k walks pixels k, k+64, ... of the box row by row; the column
-        // and row advance by 64 % w and 64 / w per step (one division per sprite)
+        // and row advance by 64 % w and 64 / w per step.  The quotients come
+        // from an approximate reciprocal: for a < 2^7 the fraction of
+        // (a + 1/2)/w is at least 1/(2w) >= 2^-12 away from an integer, far
+        // above the reciprocal's error, so the floor is exact.
         const int w = x1 - x0 + 1;
-        const int dq = 64 / w, dr = 64 - dq * w;
-        int col = lane % w, row = lane / w;
+        const float rw = __builtin_amdgcn_rcpf((float)w);
+        const int dq = (int)(64.5f * rw), dr = 64 - dq * w;
+        int row = (int)(((float)lane + 0.5f) * rw), col = lane - row * w;
         for (int k = lane; k < a; k += 64) {
             const int px = x0 + col, py = y0 + row;
             col += dr;
@@ -242,6 +246,12 @@ __global__ void __launch_bounds__(256) k_raster(const int16_t *splats, const uin
 // first, so the early depth rejection in fragment() sees the occluders first.
 constexpr int kBlock = VP_SPLAT_BLOCK;
 constexpr int kBins = 256;
+// Rasterised in phases of growing distance: the nearest eighth of the
+// visible blocks, up to half, then the rest; a Hi-Z of the frame is built
+// between phases.
+// counters: [0, kBins) bin counts, [kBins, 2 kBins) bin cursors, the total
+constexpr int kTotal = 2 * kBins;
+constexpr int kCounters = kTotal + 1;
 
 __global__ void __launch_bounds__(256) k_bounds(const int16_t *splats, const uint8_t *lod,
                                                 uint32_t lod_log2, uint64_t n, float *bounds)
@@ -283,42 +293,52 @@ struct CullPlanes {
 
 __global__ void k_blk_reset(uint32_t *cnt)
 {
-    for (int i = threadIdx.x; i < 2 * kBins + 2; i += blockDim.x) cnt[i] = 0u;
+    for (int i = threadIdx.x; i < kCounters; i += blockDim.x) cnt[i] = 0u;
 }
 
-__global__ void k_blk_cull(const float *bounds, uint64_t nb, CullPlanes cp, uint32_t *bin,
-                           uint32_t *cnt)
+__device__ __forceinline__ uint32_t block_bin(const float *bounds, uint64_t b,
+                                              const CullPlanes &cp)
 {
+    const float *o = bounds + 8 * b;
+    const float lo[3] = {o[0], o[1], o[2]}, hi[3] = {o[4], o[5], o[6]};
+    if (!(lo[0] <= hi[0])) return 0xFFFFFFFFu;   // empty block
+    for (int k = 0; k < 6; k++) {
+        // the largest value of the plane function over the box of centres
+        float f = cp.p[k][3], mag = fabsf(cp.p[k][3]);
+        for (int a = 0; a < 3; a++) {
+            const float c = cp.p[k][a];
+            f += c * (c > 0.0f ? hi[a] : lo[a]);
+            mag += fabsf(c) * fmaxf(fabsf(lo[a]), fabsf(hi[a]));
+        }
+        // the rounding of the vertex stage's clip test is far below 1e-3 of mag
+        if (f < -1e-3f * mag) return 0xFFFFFFFFu;
+    }
+    float dd = 0.0f;
+    for (int a = 0; a < 3; a++) {
+        const float e = fmaxf(fmaxf(lo[a] - o[3] - cp.eye[a], cp.eye[a] - hi[a] - o[3]), 0.0f);
+        dd += e * e;
+    }
+    const float t = sqrtf(sqrtf(dd) / cp.far_plane);
+    return (uint32_t)fminf(t * (float)kBins, (float)(kBins - 1));
+}
+
+// Bins of the blocks, counted per workgroup in LDS and added to the global
+// counts once per workgroup and bin.
+__global__ void __launch_bounds__(256) k_blk_cull(const float *bounds, uint64_t nb, CullPlanes cp,
+                                                  uint32_t *bin, uint32_t *cnt)
+{
+    __shared__ uint32_t hist[kBins];
+    for (int k = threadIdx.x; k < kBins; k += blockDim.x) hist[k] = 0u;
+    __syncthreads();
     for (uint64_t b = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; b < nb;
          b += (uint64_t)gridDim.x * blockDim.x) {
-        const float *o = bounds + 8 * b;
-        const float lo[3] = {o[0], o[1], o[2]}, hi[3] = {o[4], o[5], o[6]};
-        bool out = !(lo[0] <= hi[0]);   // empty block
-        for (int k = 0; k < 6 && !out; k++) {
-            // the largest value of the plane function over the box of centres
-            float f = cp.p[k][3], mag = fabsf(cp.p[k][3]);
-            for (int a = 0; a < 3; a++) {
-                const float c = cp.p[k][a];
-                f += c * (c > 0.0f ? hi[a] : lo[a]);
-                mag += fabsf(c) * fmaxf(fabsf(lo[a]), fabsf(hi[a]));
-            }
-            // the rounding of the vertex stage's clip test is far below 1e-3 of mag
-            out = f < -1e-3f * mag;
-        }
-        if (out) {
-            bin[b] = 0xFFFFFFFFu;
-            continue;
-        }
-        float dd = 0.0f;
-        for (int a = 0; a < 3; a++) {
-            const float e = fmaxf(fmaxf(lo[a] - o[3] - cp.eye[a], cp.eye[a] - hi[a] - o[3]), 0.0f);
-            dd += e * e;
-        }
-        const float t = sqrtf(sqrtf(dd) / cp.far_plane);
-        const uint32_t k = (uint32_t)fminf(t * (float)kBins, (float)(kBins - 1));
+        const uint32_t k = block_bin(bounds, b, cp);
         bin[b] = k;
-        atomicAdd(cnt + k, 1u);
+        if (k != 0xFFFFFFFFu) atomicAdd(hist + k, 1u);
     }
+    __syncthreads();
+    for (int k = threadIdx.x; k < kBins; k += blockDim.x)
+        if (hist[k]) atomicAdd(cnt + k, hist[k]);
 }
 
 // exclusive scan of the bin counts into the bin cursors; cnt[2*kBins] = total
@@ -335,35 +355,45 @@ __global__ void __launch_bounds__(kBins) k_blk_scan(uint32_t *cnt)
         __syncthreads();
     }
     cnt[kBins + t] = sm[t] - cnt[t];
-    if (t == kBins - 1) cnt[2 * kBins] = sm[t];
+    if (t == kBins - 1) cnt[kTotal] = sm[t];
 }
 
-__global__ void k_blk_scatter(const uint32_t *bin, uint64_t nb, uint32_t *cnt, uint32_t *order)
+// Blocks into their bins' ranges of the order: per 256 blocks, a workgroup
+// counts its blocks per bin in LDS, reserves each bin's share with one global
+// add, then places every block inside its share.
+__global__ void __launch_bounds__(256) k_blk_scatter(const uint32_t *bin, uint64_t nb,
+                                                     uint32_t *cnt, uint32_t *order)
 {
-    for (uint64_t b = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; b < nb;
-         b += (uint64_t)gridDim.x * blockDim.x) {
-        const uint32_t k = bin[b];
-        if (k == 0xFFFFFFFFu) continue;
-        order[atomicAdd(cnt + kBins + k, 1u)] = (uint32_t)b;
+    __shared__ uint32_t num[kBins], base[kBins];
+    for (uint64_t b0 = blockIdx.x * (uint64_t)blockDim.x; b0 < nb;
+         b0 += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t b = b0 + threadIdx.x;
+        for (int k = threadIdx.x; k < kBins; k += blockDim.x) num[k] = 0u;
+        __syncthreads();
+        const uint32_t k = b < nb ? bin[b] : 0xFFFFFFFFu;
+        if (k != 0xFFFFFFFFu) atomicAdd(num + k, 1u);
+        __syncthreads();
+        for (int q = threadIdx.x; q < kBins; q += blockDim.x) {
+            base[q] = num[q] ? atomicAdd(cnt + kBins + q, num[q]) : 0u;
+            num[q] = 0u;
+        }
+        __syncthreads();
+        if (k != 0xFFFFFFFFu) order[base[k] + atomicAdd(num + k, 1u)] = (uint32_t)b;
+        __syncthreads();
     }
 }
 
-// Persistent: workgroups take the ordered blocks through a ticket, so blocks
-// start nearest first.
+// Workgroup g rasterises the ordered blocks g, g + G, g + 2G, ...: every
+// workgroup starts at the near end of the order, and no counter is shared.
 __global__ void __launch_bounds__(kBlock) k_raster_blocks(const int16_t *splats,
                                                            const uint8_t *lod, uint32_t lod_log2,
                                                            uint64_t n, SplatCam c,
                                                            const float4 *rays, uint64_t *frame,
-                                                           const uint32_t *order, uint32_t *cnt)
+                                                           const uint32_t *order,
+                                                           const uint32_t *cnt)
 {
-    __shared__ uint32_t s_t;
-    const uint32_t total = cnt[2 * kBins];
-    for (;;) {
-        if (threadIdx.x == 0) s_t = atomicAdd(cnt + 2 * kBins + 1, 1u);
-        __syncthreads();
-        const uint32_t t = s_t;
-        __syncthreads();
-        if (t >= total) break;
+    const uint32_t total = cnt[kTotal];
+    for (uint32_t t = blockIdx.x; t < total; t += gridDim.x) {
         const uint64_t i = (uint64_t)order[t] * kBlock + threadIdx.x;
         raster_splat(splats, lod, lod_log2, n, i, c, rays, frame);
     }
@@ -528,7 +558,7 @@ static int splat_frame(vp_ctx *ctx, const int16_t *d_splats, const uint8_t *d_lo
     hipLaunchKernelGGL(k_rays, pg, dim3(256), 0, st, c, rays);
     hipLaunchKernelGGL(k_clear, dim3(1024), dim3(256), 0, st, frame, np);
     if (n && blk) {
-        // blk: [bin per block | order | counts (2*kBins+2)]
+        // blk: [bin per block | order | counters]
         uint32_t *bin = blk, *order = blk + nb, *cnt = blk + 2 * nb;
         const uint32_t g = (uint32_t)((nb + 255) / 256 < 4096 ? (nb + 255) / 256 : 4096);
         hipLaunchKernelGGL(k_blk_reset, dim3(1), dim3(256), 0, st, cnt);
@@ -536,8 +566,8 @@ static int splat_frame(vp_ctx *ctx, const int16_t *d_splats, const uint8_t *d_lo
                            bin, cnt);
         hipLaunchKernelGGL(k_blk_scan, dim3(1), dim3(kBins), 0, st, cnt);
         hipLaunchKernelGGL(k_blk_scatter, dim3(g), dim3(256), 0, st, bin, nb, cnt, order);
-        hipLaunchKernelGGL(k_raster_blocks, dim3((uint32_t)ncu * 8u), dim3(kBlock), 0, st, d_splats,
-                           d_lod, lod_log2, n, c, rays, frame, order, cnt);
+        hipLaunchKernelGGL(k_raster_blocks, dim3((uint32_t)ncu * 8u), dim3(kBlock), 0, st,
+                           d_splats, d_lod, lod_log2, n, c, rays, frame, order, cnt);
     } else if (n) {
         hipLaunchKernelGGL(k_raster, dim3((uint32_t)nb), dim3(256), 0, st, d_splats, d_lod,
                            lod_log2, n, c, rays, frame);

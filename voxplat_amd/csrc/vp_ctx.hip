@@ -46,8 +46,8 @@ int ensure_blocks(vp_ctx *ctx, uint64_t blocks)
 {
     if (blocks <= ctx->blk_cap) return VP_OK;
     uint32_t *p = nullptr;
-    // a bin and an order slot per block, then 2 * 256 bin counters, total, ticket
-    if (hipMalloc(&p, (2 * blocks + 2 * 256 + 2) * sizeof(uint32_t)) != hipSuccess)
+    // a bin and an order slot per block, then 2 * 256 bin counters and the total
+    if (hipMalloc(&p, (2 * blocks + 2 * 256 + 1) * sizeof(uint32_t)) != hipSuccess)
         return VP_ERR_NOMEM;
     if (ctx->d_blk) {
         if (hipDeviceSynchronize() != hipSuccess) return VP_ERR_HIP;
