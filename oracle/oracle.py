@@ -54,6 +54,9 @@ def lib() -> C.CDLL:
         "vpo_splat_render": (None, [_P, _P, C.c_uint32, C.c_uint64, C.POINTER(Camera), _P, _P,
                                     _P, _P, _P]),
         "vpo_splat_backward": (None, [C.c_uint64, C.POINTER(Camera), _P, _P, _P, _P]),
+        "vpo_chunk_make_mesh": (C.c_uint32, [W, _P, C.c_uint32, _P, C.c_uint32, _P, _P]),
+        "vpo_mesh_list": (C.c_int64, [W, _P, _P, C.c_uint32, _P, C.c_uint32, C.c_int, _P, _P,
+                                      C.c_uint64, _P]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -162,6 +165,34 @@ def remesh_list(w: WorldDesc, voxels: np.ndarray, ids: np.ndarray, smap: np.ndar
                               threads, _p(out), total, _p(offsets), _p(counts))
     assert r == total
     return out[:total].reshape(-1, 4), offsets, counts
+
+
+def chunk_make_mesh(w: WorldDesc, voxels: np.ndarray, chunk: int, smap: np.ndarray):
+    """(geometry int16[f, 4, 4], index u32[f, 6]) of one chunk."""
+    sx, _ = w.shadow_dims()
+    fmax = 3 * w.R ** 3
+    g = np.empty(16 * fmax, np.int16)
+    ix = np.empty(6 * fmax, np.uint32)
+    f = lib().vpo_chunk_make_mesh(C.byref(w), _p(voxels), chunk, _p(smap), sx, _p(g), _p(ix))
+    return g[:16 * f].reshape(f, 4, 4).copy(), ix[:6 * f].reshape(f, 6).copy()
+
+
+def mesh_list(w: WorldDesc, voxels: np.ndarray, ids: np.ndarray, smap: np.ndarray,
+              threads: int = 8):
+    """(geometry int16[F, 4, 4], index u32[F, 6], face offsets u64[n+1])."""
+    ids = np.ascontiguousarray(ids, np.uint32)
+    n = len(ids)
+    sx, _ = w.shadow_dims()
+    offs = np.zeros(n + 1, np.uint64)
+    lib().vpo_mesh_list(C.byref(w), _p(voxels), _p(ids), n, _p(smap), sx, threads, None, None,
+                        0, _p(offs))
+    total = int(offs[n])
+    g = np.empty(max(16 * total, 16), np.int16)
+    ix = np.empty(max(6 * total, 6), np.uint32)
+    r = lib().vpo_mesh_list(C.byref(w), _p(voxels), _p(ids), n, _p(smap), sx, threads, _p(g),
+                            _p(ix), total, _p(offs))
+    assert r == total
+    return g[:16 * total].reshape(total, 4, 4), ix[:6 * total].reshape(total, 6), offs
 
 
 def remesh_time(w: WorldDesc, voxels: np.ndarray, ids: np.ndarray, smap: np.ndarray,

@@ -413,6 +413,189 @@ double vpo_remesh_time(const vp_world_desc *w, const uint8_t *voxels,
 }
 
 /* ========================================================================
+ * Near-field quad mesh: chunk_make_mesh (src/chunkset/mesher.c:184-357),
+ * with its ambient-occlusion sampling (sample_ao / sample_ao_border,
+ * mesher.c:73-170) and shadow_sample / shadow_sample_normal
+ * (src/chunkset/shadow.h:52-75).  A face is emitted for every voxel A of
+ * the chunk and axis i whose +i neighbour B differs from A in occupancy
+ * (B from the +i chunk at the edge, air beyond the world), in voxel order
+ * z, y, x and then axis 0, 1, 2.  Per face: 4 vertices of int16
+ * {x, y, z, attribute} and 6 u32 indices counted from the chunk's first
+ * vertex.
+ * ====================================================================== */
+
+/* solid (nonzero) voxel at world coordinates, air beyond the world; the
+ * coordinates are u32, so -1 wraps past the end (edit.c:10-35) */
+static inline int world_solid(const vp_world_desc *w, const uint8_t *voxels,
+                              uint32_t x, uint32_t y, uint32_t z)
+{
+    uint32_t b = w->root_bitw;
+    if (x >= (1u << w->max_bitw[0]) << b || y >= (1u << w->max_bitw[1]) << b ||
+        z >= (1u << w->max_bitw[2]) << b)
+        return 0;
+    return world_voxel(w, voxels, x, y, z) != 0;
+}
+
+static inline uint8_t world_value(const vp_world_desc *w, const uint8_t *voxels,
+                                  uint32_t x, uint32_t y, uint32_t z)
+{
+    uint32_t b = w->root_bitw;
+    if (x >= (1u << w->max_bitw[0]) << b || y >= (1u << w->max_bitw[1]) << b ||
+        z >= (1u << w->max_bitw[2]) << b)
+        return 0;
+    return world_voxel(w, voxels, x, y, z);
+}
+
+/* shadow_sample with the reference's comparison M < y+1 in u32: at
+ * y = 0xFFFFFFFF (one below the world) y+1 wraps to 0 and the sample is
+ * shadowed whatever the map holds, so the map is not read there */
+static int mesh_shadow(const uint16_t *map, uint32_t size_x, const uint32_t ws[3])
+{
+    if (ws[1] == 0xFFFFFFFFu) return 1;
+    return vpo_shadow_sample(map, size_x, ws);
+}
+
+/* shadow_sample_normal, shadow.h:65-75: the entry and its -1 neighbour */
+static int mesh_shadow_normal(const uint16_t *map, uint32_t size_x, const uint32_t ws[3])
+{
+    uint32_t i = vpo_shadow_index(size_x, ws);
+    int lit = (map[i] <= ws[1]) && (map[i - 1] <= ws[1]);
+    return !lit;
+}
+
+/* corners of the unit cube's +x, +y, +z faces in vertex order (mesher.c:19-31) */
+static const uint8_t mesh_corner[3][4][3] = {
+    {{1, 0, 0}, {1, 0, 1}, {1, 1, 1}, {1, 1, 0}},
+    {{0, 1, 0}, {1, 1, 0}, {1, 1, 1}, {0, 1, 1}},
+    {{1, 0, 1}, {0, 0, 1}, {0, 1, 1}, {1, 1, 1}},
+};
+/* triangle indices by [normal][rotated] (mesher.c:54-66) */
+static const uint8_t mesh_tri[2][2][6] = {
+    {{0, 3, 1, 2, 1, 3}, {3, 2, 0, 1, 0, 2}},
+    {{1, 3, 0, 3, 1, 2}, {0, 2, 3, 2, 0, 1}},
+};
+
+uint32_t vpo_chunk_make_mesh(const vp_world_desc *w, const uint8_t *voxels, uint32_t chunk,
+                             const uint16_t *map, uint32_t size_x,
+                             int16_t *geom, uint32_t *index)
+{
+    uint32_t b = w->root_bitw, R = 1u << b, cc[3];
+    vpo_chunk_coords(w, chunk, cc);
+    const uint32_t org[3] = {cc[0] << b, cc[1] << b, cc[2] << b};
+    uint32_t nf = 0;
+    for (uint32_t z = 0; z < R; z++)
+        for (uint32_t y = 0; y < R; y++)
+            for (uint32_t x = 0; x < R; x++) {
+                const uint32_t wa[3] = {org[0] + x, org[1] + y, org[2] + z};
+                const uint8_t A = world_value(w, voxels, wa[0], wa[1], wa[2]);
+                for (int i = 0; i < 3; i++) {
+                    uint32_t wb[3] = {wa[0], wa[1], wa[2]};
+                    wb[i]++;
+                    const uint8_t B = world_value(w, voxels, wb[0], wb[1], wb[2]);
+                    if ((A == 0) == (B == 0)) continue;
+                    const int normal = (A == 0);
+                    /* the air cell of the pair and the 8 cells around it in
+                     * the plane normal to axis i */
+                    uint32_t air[3] = {wa[0], wa[1], wa[2]};
+                    air[i] += (B == 0);
+                    const int u0 = i == 0 ? 1 : 0, u1 = i == 2 ? 1 : 2;
+                    int aon[4], aoc[4];
+                    {
+                        uint32_t q[3];
+#define SOL(d0, d1) (q[0] = air[0], q[1] = air[1], q[2] = air[2], q[u0] += (uint32_t)(d0), \
+                     q[u1] += (uint32_t)(d1), world_solid(w, voxels, q[0], q[1], q[2]))
+                        aon[0] = SOL(-1, 0);
+                        aon[2] = SOL(+1, 0);
+                        aon[1] = SOL(0, -1);
+                        aon[3] = SOL(0, +1);
+                        aoc[0] = SOL(-1, -1);
+                        aoc[1] = SOL(+1, -1);
+                        aoc[2] = SOL(+1, +1);
+                        aoc[3] = SOL(-1, +1);
+#undef SOL
+                    }
+                    int cv[4], vao[4];
+                    for (int k = 0; k < 4; k++) cv[k] = (aon[k] + aon[(k + 1) & 3]) | aoc[k];
+                    if (i == 0) {
+                        vao[0] = cv[0]; vao[3] = cv[1]; vao[2] = cv[2]; vao[1] = cv[3];
+                    } else if (i == 1) {
+                        vao[0] = cv[0]; vao[1] = cv[1]; vao[2] = cv[2]; vao[3] = cv[3];
+                    } else {
+                        vao[1] = cv[0]; vao[0] = cv[1]; vao[3] = cv[2]; vao[2] = cv[3];
+                    }
+                    const int rot = (vao[0] + vao[2] < vao[1] + vao[3]);
+                    int shadow = 0, diamond = 3;
+                    if (i == 0 && normal) {
+                        shadow = mesh_shadow_normal(map, size_x, wb);   /* the solid cell */
+                    } else if (i == 1 && !normal) {
+                        shadow = mesh_shadow(map, size_x, wa);
+                    } else if (i == 2) {
+                        uint32_t s[3] = {wa[0], wa[1], wa[2]};
+                        if (!B) s[2] += 1;
+                        diamond = (!mesh_shadow(map, size_x, s)) << 1;
+                        s[1] -= 1;
+                        diamond |= !mesh_shadow(map, size_x, s);
+                    }
+                    for (int v = 0; v < 4; v++) {
+                        int16_t *g = geom + 16 * (size_t)nf + 4 * v;
+                        for (int a = 0; a < 3; a++)
+                            g[a] = (int16_t)(wa[a] + mesh_corner[i][v][a]);
+                        g[3] = (int16_t)((A | B) | (vao[v] << 6) | ((i + 3 * normal) << 8) |
+                                         (v << 11) | (shadow << 13) | (diamond << 14));
+                    }
+                    for (int k = 0; k < 6; k++)
+                        index[6 * (size_t)nf + k] = 4 * nf + mesh_tri[normal][rot][k];
+                    nf++;
+                }
+            }
+    return nf;
+}
+
+int64_t vpo_mesh_list(const vp_world_desc *w, const uint8_t *voxels, const uint32_t *ids,
+                      uint32_t n, const uint16_t *map, uint32_t size_x, int threads,
+                      int16_t *geom_out, uint32_t *index_out, uint64_t cap_faces,
+                      uint64_t *face_offsets)
+{
+    uint32_t R = 1u << w->root_bitw;
+    uint64_t fmax = 3ull * R * R * R;
+    int16_t **gl = (int16_t **)calloc(n ? n : 1, sizeof(int16_t *));
+    uint32_t **il = (uint32_t **)calloc(n ? n : 1, sizeof(uint32_t *));
+    uint32_t *nfs = (uint32_t *)calloc(n ? n : 1, sizeof(uint32_t));
+    if (threads < 1) threads = 1;
+#pragma omp parallel num_threads(threads)
+    {
+        int16_t *g = (int16_t *)malloc(fmax * 16 * sizeof(int16_t));
+        uint32_t *ix = (uint32_t *)malloc(fmax * 6 * sizeof(uint32_t));
+#pragma omp for schedule(dynamic, 1)
+        for (int64_t k = 0; k < (int64_t)n; k++) {
+            uint32_t nf = vpo_chunk_make_mesh(w, voxels, ids[k], map, size_x, g, ix);
+            nfs[k] = nf;
+            gl[k] = (int16_t *)malloc((nf ? nf : 1) * 16 * sizeof(int16_t));
+            il[k] = (uint32_t *)malloc((nf ? nf : 1) * 6 * sizeof(uint32_t));
+            memcpy(gl[k], g, (size_t)nf * 16 * sizeof(int16_t));
+            memcpy(il[k], ix, (size_t)nf * 6 * sizeof(uint32_t));
+        }
+        free(g); free(ix);
+    }
+    uint64_t off = 0;
+    int ok = 1;
+    for (uint32_t k = 0; k < n; k++) {
+        face_offsets[k] = off;
+        if (geom_out && off + nfs[k] <= cap_faces) {
+            memcpy(geom_out + 16 * off, gl[k], (size_t)nfs[k] * 16 * sizeof(int16_t));
+            memcpy(index_out + 6 * off, il[k], (size_t)nfs[k] * 6 * sizeof(uint32_t));
+        } else if (nfs[k]) {
+            ok = 0;
+        }
+        off += nfs[k];
+        free(gl[k]); free(il[k]);
+    }
+    face_offsets[n] = off;
+    free(gl); free(il); free(nfs);
+    return ok ? (int64_t)off : -1;
+}
+
+/* ========================================================================
  * RLE, src/chunkset/rle.c:44-116
  * ====================================================================== */
 uint32_t vpo_rle_compress(const uint8_t *data, uint32_t len, uint32_t *words, uint32_t cap)

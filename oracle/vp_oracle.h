@@ -77,6 +77,19 @@ double   vpo_remesh_time(const vp_world_desc *w, const uint8_t *voxels,
                          const uint16_t *map, uint32_t size_x, int threads,
                          uint64_t *total_items);
 
+/* ---- near-field quad mesh (mesher.c:184-357) ------------------------------ */
+/* One chunk: returns the face count f; geom gets 16 int16 per face (4
+ * vertices of x, y, z, attribute), index 6 u32 per face (chunk-local). */
+uint32_t vpo_chunk_make_mesh(const vp_world_desc *w, const uint8_t *voxels, uint32_t chunk,
+                             const uint16_t *map, uint32_t size_x,
+                             int16_t *geom, uint32_t *index);
+/* A list of chunks, concatenated in list order; face_offsets[n+1]. Returns
+ * the face total, -1 if it exceeds cap_faces (offsets are still written). */
+int64_t  vpo_mesh_list(const vp_world_desc *w, const uint8_t *voxels, const uint32_t *ids,
+                       uint32_t n, const uint16_t *map, uint32_t size_x, int threads,
+                       int16_t *geom_out, uint32_t *index_out, uint64_t cap_faces,
+                       uint64_t *face_offsets);
+
 /* ---- RLE (rle.c:44-116) -------------------------------------------------- */
 /* returns words written (terminator included), 0 if cap is too small */
 uint32_t vpo_rle_compress(const uint8_t *data, uint32_t len, uint32_t *words, uint32_t cap);

@@ -134,3 +134,96 @@ def random_world(w, rng, density: float, colours=(1, 64)) -> np.ndarray:
     solid = rng.random(n) < density
     col = rng.integers(colours[0], colours[1], n).astype(np.uint8)
     return np.where(solid, col, 0).astype(np.uint8)
+
+
+# ---- near-field quad mesh ---------------------------------------------------
+# Corners of the unit cube's +x, +y, +z faces in vertex order (the vertex
+# table of mesher.c:19-31) and triangle index rows by [normal][rotated]
+# (mesher.c:54-66): data, not derived.
+MESH_CORNER = np.array([
+    [[1, 0, 0], [1, 0, 1], [1, 1, 1], [1, 1, 0]],
+    [[0, 1, 0], [1, 1, 0], [1, 1, 1], [0, 1, 1]],
+    [[1, 0, 1], [0, 0, 1], [0, 1, 1], [1, 1, 1]],
+], np.int64)
+MESH_TRI = np.array([[[0, 3, 1, 2, 1, 3], [3, 2, 0, 1, 0, 2]],
+                     [[1, 3, 0, 3, 1, 2], [0, 2, 3, 2, 0, 1]]], np.int64)
+
+
+def _shadow_u32(smap, sx, wx, wy, wz, side):
+    """shadow_sample (side +1) / shadow_sample_normal (side -1) with the
+    reference's u32 arithmetic: lit iff both entries are < (wy + 1) mod 2^32,
+    so a sample at wy = 2^32 - 1 is shadowed whatever the map holds."""
+    M32 = (1 << 32) - 1
+    wx, wy, wz = (np.asarray(a, np.int64) & M32 for a in (wx, wy, wz))
+    top = (wy + 1) & M32
+    i = (wx + wy + sx * wz) & M32
+    live = top != 0
+    i = np.where(live, i, 1)
+    a = smap[i].astype(np.int64)
+    b = smap[i + side].astype(np.int64)
+    lit = live & (a < top) & (b < top)
+    return (~lit).astype(np.int64) & 1
+
+
+def mesh_chunk(w, world: np.ndarray, chunk: int, smap: np.ndarray):
+    """(geometry int16[f, 4, 4], index u32[f, 6]) of one chunk, vectorised:
+    faces are the occupancy changes between each voxel and its +axis
+    neighbour, ordered (z, y, x, axis) by np.nonzero over a [z, y, x, axis]
+    array; the ambient term of each vertex is read geometrically, from the
+    two edge cells and the corner cell next to that corner in the face's air
+    cell plane (value 0..3 = (edge + edge) | corner)."""
+    R = w.R
+    cx, cy, cz = chunk_coords(w, chunk)
+    sx, _ = w.shadow_dims()
+    P = np.pad(world, 1)                        # air around the world, shift +1
+    o = np.array([cx * R, cy * R, cz * R])
+    Wd = P[o[2]:o[2] + R + 3, o[1]:o[1] + R + 3, o[0]:o[0] + R + 3]   # world o-1 ..
+    S = (Wd != 0).astype(np.int64)
+    c = (slice(1, R + 1),) * 3
+    A = Wd[c]
+    Bn = [Wd[1:R + 1, 1:R + 1, 2:R + 2], Wd[1:R + 1, 2:R + 2, 1:R + 1], Wd[2:R + 2, 1:R + 1, 1:R + 1]]
+    F = np.stack([(A != 0) != (B != 0) for B in Bn], -1)
+    z, y, x, ax = np.nonzero(F)
+    f = len(z)
+    a = A[z, y, x].astype(np.int64)
+    b = np.stack(Bn, -1)[z, y, x, ax].astype(np.int64)
+    normal = (a == 0).astype(np.int64)
+    loc = np.stack([x, y, z], 1)                           # local xyz
+    e = np.eye(3, dtype=np.int64)[ax]
+    air = loc + 1 + e * (b == 0)[:, None]                  # window coords of the air cell
+    ws = loc + o                                           # world coords of A
+    geom = np.zeros((f, 4, 4), np.int64)
+    ao = np.zeros((f, 4), np.int64)
+
+    def solid(p):
+        return S[p[:, 2], p[:, 1], p[:, 0]]
+    for v in range(4):
+        corner = MESH_CORNER[ax, v]                        # [f, 3]
+        d = (2 * corner - 1) * (1 - e)                     # in-plane step towards the corner
+        edges = np.zeros(f, np.int64)
+        for k in range(3):
+            dk = np.zeros_like(d)
+            dk[:, k] = d[:, k]
+            edges += solid(air + dk) * (d[:, k] != 0)
+        ao[:, v] = edges | solid(air + d)
+        geom[:, v, :3] = ws + corner
+    rot = (ao[:, 0] + ao[:, 2] < ao[:, 1] + ao[:, 3]).astype(np.int64)
+    shadow = np.zeros(f, np.int64)
+    diamond = np.full(f, 3, np.int64)
+    sel = (ax == 0) & (normal == 1)                        # the solid +x cell, entry and -1
+    p = ws[sel] + np.array([1, 0, 0])
+    shadow[sel] = _shadow_u32(smap, sx, p[:, 0], p[:, 1], p[:, 2], -1)
+    sel = (ax == 1) & (normal == 0)
+    p = ws[sel]
+    shadow[sel] = _shadow_u32(smap, sx, p[:, 0], p[:, 1], p[:, 2], +1)
+    sel = ax == 2
+    p = ws[sel] + np.array([0, 0, 1]) * (b[sel] == 0)[:, None]
+    hi = 1 - _shadow_u32(smap, sx, p[:, 0], p[:, 1], p[:, 2], +1)
+    lo = 1 - _shadow_u32(smap, sx, p[:, 0], p[:, 1] - 1, p[:, 2], +1)
+    diamond[sel] = hi << 1 | lo
+    for v in range(4):
+        geom[:, v, 3] = ((a | b) | ao[:, v] << 6 | (ax + 3 * normal) << 8 | v << 11 |
+                         shadow << 13 | diamond << 14)
+    index = 4 * np.arange(f, dtype=np.int64)[:, None] + MESH_TRI[normal, rot]
+    return (geom.astype(np.uint16).view(np.int16).reshape(f, 4, 4),
+            index.astype(np.uint32))
