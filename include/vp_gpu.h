@@ -13,6 +13,9 @@
  *                        + chunk_make_splatlist (L0..L4)  src/chunkset/mesher.c:497-536
  *                        + chunk_mask_downsample (L1..L4) src/chunkset/mesher.c:460-493
  *                        + shadow_sample                  src/chunkset/shadow.h:45-63
+ *   vp_gpu_mesh       <- chunk_make_mesh                  src/chunkset/mesher.c:184-357
+ *                        + sample_ao / sample_ao_border   src/chunkset/mesher.c:72-171
+ *                        + shadow_sample(_normal)         src/chunkset/shadow.h:52-75
  *   vp_gpu_make_mask  <- chunk_make_mask                  src/chunkset/mesher.c:377-456
  *   vp_gpu_lod_mip    <- chunk_mask_downsample            src/chunkset/mesher.c:460-493
  *   vp_gpu_rle        <- rle_compress / rle_decompress    src/chunkset/rle.c:44-116
@@ -120,6 +123,40 @@ int vp_gpu_cull_mesh(vp_ctx *ctx, const vp_world_desc *world,
                      const vp_shadow_desc *shadow,
                      const uint32_t *d_chunk_ids, uint32_t n, const uint32_t *d_n,
                      uint32_t flags, const vp_splat_out *out, void *stream);
+
+/* ---- near-field quad mesh ---------------------------------------------------
+ * Replaces chunk_make_mesh (src/chunkset/mesher.c:184-357), one call per
+ * chunk in the reference, with one launch over a chunk list.
+ * Per chunk, faces in the reference's order (voxels z, y, x; axis 0, 1, 2):
+ * one face wherever a voxel A of the chunk and its +axis neighbour B differ
+ * in occupancy (B from the next chunk at the edge, air beyond the world).
+ * Face f of the concatenation: d_geometry[16 f ..] = 4 vertices
+ * {x, y, z, attribute} as int16, attribute bits __SUVNNN AORRGGBB
+ * (mesher.c:332-342); d_index[6 f ..] = two triangles, counted from the
+ * chunk's first vertex as the reference's per-chunk index buffer.
+ * The reference's geometry_items / index_items of list entry k are
+ * 16 * faces_k and 6 * faces_k, faces_k = d_face_offsets[k+1] - d_face_offsets[k].
+ * Shadow samples read the map as vp_shadow_desc describes (zero padding
+ * past the world: a z face on the world's far z plane samples a lit row). */
+typedef struct {
+    int16_t  *d_geometry;         /* capacity_faces * 16 int16                     */
+    uint32_t *d_index;            /* capacity_faces * 6 u32                        */
+    uint64_t  capacity_faces;
+    uint64_t *d_face_offsets;     /* n+1 exclusive prefix, in faces                */
+} vp_mesh_out;
+
+/* Mesh the chunks d_chunk_ids[0..n), concatenated in list order.
+ * d_geometry = d_index = NULL with capacity_faces = 0 counts only: the
+ * offsets are written, nothing else.  Chunks whose faces do not fit are not
+ * written (their offsets are) and VP_STATUS_OVERFLOW is raised; ids outside
+ * the world get no faces and raise VP_STATUS_BAD_ID.
+ * d_uniform (nullable): per chunk, -1 = mixed, else every voxel's value;
+ * uniform chunks and uniform neighbours are not read.
+ * Supported roots: 4..64 (root_bitw 2..6). */
+int vp_gpu_mesh(vp_ctx *ctx, const vp_world_desc *world, const uint8_t *d_voxels,
+                const int16_t *d_uniform, const vp_shadow_desc *shadow,
+                const uint32_t *d_chunk_ids, uint32_t n, const vp_mesh_out *out,
+                void *stream);
 
 /* Level-0 visibility masks ((R+1)^3 bytes per chunk, index (z*(R+1)+y)*(R+1)+x)
  * exactly as chunk_make_mask leaves them in a zeroed buffer. */

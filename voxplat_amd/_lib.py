@@ -132,6 +132,47 @@ class GPU:
             C.byref(so), st), "vp_gpu_cull_mesh")
         return recs, offsets, counts
 
+    def mesh(self, world: abi.WorldDesc, voxels, ids, shadow_map, uniform=None,
+             capacity: int | None = None, count_only: bool = False, out=None, stream=None):
+        """Near-field quad mesh of chunks `ids` (int32 tensor).  Returns
+        (geometry int16[F,4,4] or None, index int32[F,6] or None, face offsets
+        int64[n+1]).  capacity=None sizes the output with a counting pass;
+        out=(geometry, index, offsets) reuses buffers."""
+        torch = self.torch
+        n = int(ids.numel())
+        dev = voxels.device
+        sh = abi.ShadowDesc()
+        sx, sz = world.shadow_dims()
+        sh.d_map, sh.size_x, sh.size_z = shadow_map.data_ptr(), sx, sz
+        st = C.c_void_p(self._stream(stream))
+        if out is None:
+            geom = index = None
+            offsets = torch.empty(n + 1, dtype=torch.int64, device=dev)
+        else:
+            geom, index, offsets = out
+        mo = abi.MeshOut()
+        mo.d_face_offsets = offsets.data_ptr()
+
+        def run(what):
+            _check(self.lib.vp_gpu_mesh(
+                self.ctx, C.byref(world), C.c_void_p(voxels.data_ptr()),
+                C.c_void_p(_ptr(uniform)), C.byref(sh), C.c_void_p(ids.data_ptr()), n,
+                C.byref(mo), st), what)
+        if count_only or (capacity is None and geom is None):
+            mo.d_geometry, mo.d_index, mo.capacity_faces = None, None, 0
+            run("vp_gpu_mesh(count)")
+            if count_only:
+                return None, None, offsets
+            capacity = int(offsets[n].item())
+        if geom is None:
+            geom = torch.empty((max(capacity, 1), 4, 4), dtype=torch.int16, device=dev)
+            index = torch.empty((max(capacity, 1), 6), dtype=torch.int32, device=dev)
+        else:
+            capacity = min(int(geom.shape[0]), int(index.shape[0])) if capacity is None else capacity
+        mo.d_geometry, mo.d_index, mo.capacity_faces = geom.data_ptr(), index.data_ptr(), capacity
+        run("vp_gpu_mesh")
+        return geom, index, offsets
+
     def make_mask(self, world: abi.WorldDesc, voxels, ids, uniform=None, stream=None):
         torch = self.torch
         n = int(ids.numel())

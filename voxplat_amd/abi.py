@@ -65,6 +65,11 @@ class SplatOut(C.Structure):
                 ("d_chunk_offsets", C.c_void_p), ("d_level_counts", C.c_void_p)]
 
 
+class MeshOut(C.Structure):
+    _fields_ = [("d_geometry", C.c_void_p), ("d_index", C.c_void_p),
+                ("capacity_faces", C.c_uint64), ("d_face_offsets", C.c_void_p)]
+
+
 class Camera(C.Structure):
     """vp_camera: column-major matrices as uploaded by gfx_svl_draw (vsplat.c:126-158)."""
     _fields_ = [("view_translation", C.c_float * 16), ("proj_view_rot", C.c_float * 16),
@@ -87,6 +92,9 @@ PROTOTYPES = {
     "vp_gpu_cull_mesh": (C.c_int, [C.c_void_p, C.POINTER(WorldDesc), C.c_void_p, C.c_void_p,
                                    C.POINTER(ShadowDesc), C.c_void_p, C.c_uint32, C.c_void_p,
                                    C.c_uint32, C.POINTER(SplatOut), C.c_void_p]),
+    "vp_gpu_mesh": (C.c_int, [C.c_void_p, C.POINTER(WorldDesc), C.c_void_p, C.c_void_p,
+                              C.POINTER(ShadowDesc), C.c_void_p, C.c_uint32,
+                              C.POINTER(MeshOut), C.c_void_p]),
     "vp_gpu_make_mask": (C.c_int, [C.c_void_p, C.POINTER(WorldDesc), C.c_void_p, C.c_void_p,
                                    C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p]),
     "vp_gpu_lod_mip": (C.c_int, [C.c_void_p, C.c_uint32, C.c_int, C.c_void_p, C.c_void_p,
