@@ -1,6 +1,7 @@
 """Plain launches of the batched remesh on a generated world, for profilers.
 
     python -m voxplat_amd.remesh_run [--world 2k|16k] [--launches K] [--copy-gib G]
+                                     [--kernel cull|mesh]
 
 Generates the world on the GPU, orders the chunk list heaviest first and
 sizes the output with counting passes, then issues K direct (not
@@ -8,7 +9,8 @@ graph-captured) launches of vp_gpu_cull_mesh over every chunk, so that a
 kernel tracer or counter collector sees one dispatch per launch.
 --copy-gib G first runs one device-to-device copy of G GiB: a kernel of
 known traffic (G GiB read, G GiB written) against which HBM byte counters
-are scaled.
+are scaled.  --kernel mesh issues vp_gpu_mesh (the near-field quad mesh)
+over every chunk in index order instead, output sized by a counting call.
 """
 from __future__ import annotations
 
@@ -21,7 +23,7 @@ import torch
 from . import abi
 from . import workload as W
 from ._lib import GPU
-from .world_frame import heaviest_first
+from .world_frame import heaviest_first, mesh_bytes
 
 
 def main(argv=None) -> int:
@@ -30,6 +32,7 @@ def main(argv=None) -> int:
     ap.add_argument("--launches", type=int, default=3)
     ap.add_argument("--copy-gib", type=float, default=0.0)
     ap.add_argument("--autotune", action="store_true", help="pick the launch shape first")
+    ap.add_argument("--kernel", choices=("cull", "mesh"), default="cull")
     a = ap.parse_args(argv)
     gpu = GPU(0)
     if a.copy_gib:
@@ -49,6 +52,8 @@ def main(argv=None) -> int:
     smap = torch.empty(abi.shadow_padded_len(sx, sz), dtype=torch.int16, device="cuda")
     gpu.shadow_build(w, vox, smap)
     ids = torch.arange(w.chunks, dtype=torch.int32, device="cuda")
+    if a.kernel == "mesh":
+        return run_mesh(gpu, cfg, w, vox, uni, smap, ids, a.launches)
     ids, offs, counts = heaviest_first(gpu, w, vox, ids, smap, uni)
     torch.cuda.synchronize()
     cap = int(offs[-1].item()) // 4
@@ -69,6 +74,31 @@ def main(argv=None) -> int:
     st = gpu.status()
     print(json.dumps({"world": cfg.name, "chunks": w.chunks, "records": cap,
                       "launch_ms": [round(x, 4) for x in ms], "status": st}))
+    gpu.close()
+    return 0 if st == 0 else 1
+
+
+def run_mesh(gpu, cfg, w, vox, uni, smap, ids, launches: int) -> int:
+    _, _, offs = gpu.mesh(w, vox, ids, smap, uniform=uni, count_only=True)
+    torch.cuda.synchronize()
+    faces = int(offs[-1].item())
+    geom = torch.empty((max(faces, 1), 4, 4), dtype=torch.int16, device="cuda")
+    index = torch.empty((max(faces, 1), 6), dtype=torch.int32, device="cuda")
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    ms = []
+    for _ in range(launches):
+        e0.record()
+        gpu.mesh(w, vox, ids, smap, uniform=uni, capacity=faces, out=(geom, index, offs))
+        e1.record()
+        e1.synchronize()
+        ms.append(e0.elapsed_time(e1))
+    st = gpu.status()
+    codes = ((geom[:faces, 0, 3].to(torch.int32) >> 8) & 7).cpu().numpy()
+    algo = mesh_bytes(uni.cpu().numpy(), codes, w.R)
+    best = min(ms)
+    print(json.dumps({"world": cfg.name, "kernel": "k_mesh", "chunks": w.chunks, "faces": faces,
+                      "launch_ms": [round(x, 4) for x in ms], "algorithmic_bytes": algo,
+                      "gb_s_best": round(algo / (best * 1e-3) / 1e9, 1), "status": st}))
     gpu.close()
     return 0 if st == 0 else 1
 

@@ -186,6 +186,21 @@ def remesh_bytes(uniform_h: np.ndarray, counts_h: np.ndarray) -> int:
     return mixed * (262144 + 3 * 4096) + uni * 3 * 4096 + 12 * records + 20 * len(uniform_h)
 
 
+def mesh_bytes(uniform_h: np.ndarray, face_codes: np.ndarray, R: int = 64) -> int:
+    """Algorithmic bytes of one quad-mesh launch: every mixed chunk's voxels
+    and every chunk's one-voxel shell ((R+2)^3 - R^3 bytes) read once, 56 B
+    written per face (4 vertices of 4 int16, 6 u32 indices), and the shadow
+    map entries each face samples: 4 B for x faces towards +x (normal) and y
+    faces of solid voxels, 8 B for z faces.  face_codes: per-face
+    axis + 3 * normal (attribute bits 8-10)."""
+    mixed = int((uniform_h < 0).sum())
+    n = len(uniform_h)
+    faces = len(face_codes)
+    counts = np.bincount(face_codes, minlength=6)
+    maps = 4 * (int(counts[3]) + int(counts[1])) + 8 * (int(counts[2]) + int(counts[5]))
+    return mixed * R ** 3 + n * ((R + 2) ** 3 - R ** 3) + 56 * faces + maps + 8 * n
+
+
 def rle_bytes(n_chunks: int, words: int, R3: int = 262144) -> int:
     """Algorithmic bytes of one RLE encode: every voxel read once, every word
     (terminators included) written once, one u64 offset and i16 value per chunk."""
