@@ -9,6 +9,7 @@ captured in a hipGraph.  Inputs are resident in HBM before timing starts.
 Also measured (not the headline value):
   * the plain per-chunk path: one launch per chunk from the C host loop;
   * splat resolve of the level-0 list at 1920x1080 over 16 seeded poses;
+  * the near-field quad mesh (chunk_make_mesh) of every chunk of the world;
   * the roofline of the remesh kernel from HIP events around the timed steps;
   * the CPU oracle (4 OpenMP threads, as chunkset_manage's team) on the same
     4096 chunks.
@@ -118,6 +119,7 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-splat", action="store_true")
+    ap.add_argument("--no-mesh", action="store_true", help="skip the quad-mesh leg")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=4)
     ap.add_argument("--workload", choices=("world2k", "world16k"), default="world2k",
@@ -266,6 +268,38 @@ def main() -> int:
                      "pixels_hit": round(sum(hits) / len(hits), 3)}
         log(f"splat: {n0} splats, {fr_ms:.2f} ms/frame -> {n0 / (fr_ms * 1e-3) / 1e9:.2f} Gsplats/s")
 
+    # ---- near-field quad mesh of the same world (every chunk, index order) ----
+    mesh_res = None
+    if not args.no_mesh:
+        from voxplat_amd.world_frame import mesh_bytes
+
+        mids = torch.arange(nch, dtype=torch.int32, device=dev)
+        _, _, moffs = gpu.mesh(w, vox, mids, smap, uniform=uni, count_only=True)
+        torch.cuda.synchronize()
+        nf = int(moffs[-1].item())
+        mgeom = torch.empty((nf, 4, 4), dtype=torch.int16, device=dev)
+        mindex = torch.empty((nf, 6), dtype=torch.int32, device=dev)
+        mout = (mgeom, mindex, moffs)
+        gpu.mesh(w, vox, mids, smap, uniform=uni, capacity=nf, out=mout)
+        torch.cuda.synchronize()
+        reps = 5
+        e0.record(stream)
+        for _ in range(reps):
+            gpu.mesh(w, vox, mids, smap, uniform=uni, capacity=nf, out=mout)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        assert gpu.status() == 0
+        m_ms = e0.elapsed_time(e1) / reps
+        codes = ((mgeom[:, 0, 3].to(torch.int32) >> 8) & 7).cpu().numpy()
+        m_algo = mesh_bytes(uni_h, codes, w.R)
+        m_gbs = m_algo / (m_ms * 1e-3) / 1e9
+        mesh_res = {"faces": nf, "ms_per_launch": round(m_ms, 4),
+                    "gvoxels_s": round(nch * VOXELS_PER_CHUNK / (m_ms * 1e-3) / 1e9, 2),
+                    "algorithmic_bytes": m_algo, "achieved_gb_s": round(m_gbs, 1),
+                    "frac_hbm": round(m_gbs / HBM_PEAK_GBS, 4)}
+        log(f"mesh: {nf} faces, {m_ms:.3f} ms/launch -> {m_gbs:.0f} GB/s algorithmic")
+        del mgeom, mindex, mout
+
     # ---- CPU baseline: the oracle on host cores ----
     cpu = None
     if not args.no_cpu:
@@ -305,6 +339,7 @@ def main() -> int:
                       "identical_output": same},
         "speedup_vs_per_chunk": round(gvox / pc_gvox, 2),
         "splat": splat_res,
+        "mesh": mesh_res,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
