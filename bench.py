@@ -227,6 +227,7 @@ def main() -> int:
 
     # ---- splat resolve at 1080p (level-0 list, chunk order) ----
     splat_res = None
+    splat_cpu = None
     if not args.no_splat:
         # the level-0 records in chunk-index order
         ids_h = ids.cpu().numpy()
@@ -267,6 +268,7 @@ def main() -> int:
                      "ms_per_frame": round(fr_ms, 3), "splats_per_s": round(n0 / (fr_ms * 1e-3), 1),
                      "pixels_hit": round(sum(hits) / len(hits), 3)}
         log(f"splat: {n0} splats, {fr_ms:.2f} ms/frame -> {n0 / (fr_ms * 1e-3) / 1e9:.2f} Gsplats/s")
+        splat_cpu = (l0[::32].contiguous().cpu().numpy(), cams[0])   # every 32nd splat: ~4 M
 
     # ---- near-field quad mesh of the same world (every chunk, index order) ----
     mesh_res = None
@@ -316,6 +318,26 @@ def main() -> int:
                          f"{args.cpu_threads} OpenMP threads as chunkset_manage "
                          f"(host has {os.cpu_count()} CPUs); {t:.2f} s"}
         log(f"cpu: {t:.2f} s -> {cpu['value']} Gvoxel/s")
+        # the other legs' oracle on bounded samples of the same inputs
+        if mesh_res is not None:
+            mixed_ids = np.nonzero(uni_h < 0)[0].astype(np.uint32)
+            msample = mixed_ids[:: max(1, len(mixed_ids) // 256)][:256]
+            t0 = time.perf_counter()
+            O.mesh_list(w, vh, msample, mh, threads=args.cpu_threads)
+            tm = time.perf_counter() - t0
+            mesh_res["cpu"] = {"gvoxels_s": round(len(msample) * VOXELS_PER_CHUNK / tm / 1e9, 4),
+                               "cores": args.cpu_threads, "kind": "port",
+                               "sample": f"{len(msample)} mixed chunks, {tm:.2f} s"}
+            log(f"cpu mesh: {len(msample)} chunks in {tm:.2f} s")
+        if splat_cpu is not None:
+            l0h, cam0 = splat_cpu
+            t0 = time.perf_counter()
+            O.splat_render(l0h, cam0)
+            ts = time.perf_counter() - t0
+            splat_res["cpu"] = {"splats_s": round(len(l0h) / ts, 1), "cores": 1, "kind": "port",
+                                "sample": f"every 32nd splat of the list ({len(l0h)}), first pose, 1920x1080, "
+                                          f"{ts:.2f} s"}
+            log(f"cpu splat: {len(l0h)} splats in {ts:.2f} s")
 
     graph.close()
     line = {
