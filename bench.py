@@ -302,6 +302,35 @@ def main() -> int:
         log(f"mesh: {nf} faces, {m_ms:.3f} ms/launch -> {m_gbs:.0f} GB/s algorithmic")
         del mgeom, mindex, mout
 
+    # ---- config 0: one 64^3 chunk, the plain per-chunk path (n = 1) ----
+    c0w = W.CFG0.desc()
+    c0vox = torch.empty(c0w.R ** 3, dtype=torch.uint8, device=dev)
+    gpu.generate(c0w, W.CFG0.gen(), c0vox)
+    c0sx, c0sz = c0w.shadow_dims()
+    c0map = torch.empty(abi.shadow_padded_len(c0sx, c0sz), dtype=torch.int16, device=dev)
+    gpu.shadow_build(c0w, c0vox, c0map)
+    c0ids = torch.zeros(1, dtype=torch.int32, device=dev)
+    _, c0offs, c0counts = gpu.cull_mesh(c0w, c0vox, c0ids, shadow_map=c0map, count_only=True)
+    torch.cuda.synchronize()
+    c0cap = int(c0offs[-1].item()) // 4
+    c0out = torch.empty((max(c0cap, 1), 4), dtype=torch.int16, device=dev)
+    c0reps = 50
+    for _ in range(5):
+        gpu.cull_mesh(c0w, c0vox, c0ids, shadow_map=c0map, capacity=c0cap, out=(c0offs, c0counts),
+                      records=c0out)
+    torch.cuda.synchronize()
+    e0.record(stream)
+    for _ in range(c0reps):
+        gpu.cull_mesh(c0w, c0vox, c0ids, shadow_map=c0map, capacity=c0cap, out=(c0offs, c0counts),
+                      records=c0out)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    assert gpu.status() == 0
+    c0_us = e0.elapsed_time(e1) * 1e3 / c0reps
+    cfg0_res = {"workload": "cull + L0..L4 of one 64^3 chunk, one launch", "records": c0cap,
+                "gpu_us_per_chunk": round(c0_us, 2)}
+    log(f"cfg0: one chunk, {c0cap} records, {c0_us:.1f} us per launch")
+
     # ---- CPU baseline: the oracle on host cores ----
     cpu = None
     if not args.no_cpu:
@@ -318,6 +347,14 @@ def main() -> int:
                          f"{args.cpu_threads} OpenMP threads as chunkset_manage "
                          f"(host has {os.cpu_count()} CPUs); {t:.2f} s"}
         log(f"cpu: {t:.2f} s -> {cpu['value']} Gvoxel/s")
+        ncores = min(16, os.cpu_count() or 1)
+        t_all, _ = O.remesh_time(w, vh, idsh, mh, ncores)
+        cpu["all_cores"] = {"value": round(nch * VOXELS_PER_CHUNK / t_all / 1e9, 4), "cores": ncores,
+                            "seconds": round(t_all, 3)}
+        log(f"cpu, {ncores} threads: {t_all:.2f} s")
+        c0v, c0m = c0vox.cpu().numpy(), c0map.cpu().numpy().view(np.uint16)
+        c0t = sorted(O.remesh_time(c0w, c0v, np.zeros(1, np.uint32), c0m, 1)[0] for _ in range(5))[2]
+        cfg0_res["cpu_us_per_chunk_1_thread"] = round(c0t * 1e6, 1)
         # the other legs' oracle on bounded samples of the same inputs
         if mesh_res is not None:
             mixed_ids = np.nonzero(uni_h < 0)[0].astype(np.uint32)
@@ -362,6 +399,7 @@ def main() -> int:
         "speedup_vs_per_chunk": round(gvox / pc_gvox, 2),
         "splat": splat_res,
         "mesh": mesh_res,
+        "cfg0": cfg0_res,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

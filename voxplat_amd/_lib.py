@@ -95,7 +95,7 @@ class GPU:
     # ---- remesh --------------------------------------------------------------
     def cull_mesh(self, world: abi.WorldDesc, voxels, ids, shadow_map=None, uniform=None,
                   lod: bool = True, v2: bool = False, capacity: int | None = None,
-                  count_only: bool = False, d_n=None, out=None, stream=None):
+                  count_only: bool = False, d_n=None, out=None, records=None, stream=None):
         """Remesh chunks `ids` (int32 tensor). Returns (records int16[N,4] or None,
         offsets int64[n+1] in int16 items, level_counts int32[n,5])."""
         torch = self.torch
@@ -124,7 +124,9 @@ class GPU:
             if count_only:
                 return None, offsets, counts
             capacity = int(offsets[n].item()) // 4
-        recs = torch.empty((max(capacity, 1), 4), dtype=torch.int16, device=dev)
+        recs = records if records is not None else \
+            torch.empty((max(capacity, 1), 4), dtype=torch.int16, device=dev)
+        assert recs.shape[0] >= capacity
         so.d_splats, so.capacity_records = recs.data_ptr(), capacity
         _check(self.lib.vp_gpu_cull_mesh(
             self.ctx, C.byref(world), C.c_void_p(voxels.data_ptr()), C.c_void_p(_ptr(uniform)),
