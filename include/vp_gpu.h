@@ -23,6 +23,10 @@
  *   vp_gpu_splat      <- gfx_svl_draw + splat.vert/.frag  src/gfx/vsplat.c:104-193,
  *                        resources/splat.vert.glsl:34-54, resources/splat.frag.glsl:60-156
  *   vp_gpu_splat_backward -- no counterpart in the reference (colour fitting).
+ *   vp_gpu_svl_nodes  <- gfx_update_svl (node uploads)   src/gfx/vsplat.c:197-338
+ *   vp_gpu_draw_gather <- gfx_svl_draw's draw queue       src/gfx/vsplat.c:163-183
+ *                        (queue built by octree_cull,     src/game.c:319-395, in
+ *                        vp_host_octree_cull, include/vp_host.h)
  *   vp_gpu_splat_bounds / vp_gpu_splat_blocks -- per-block culling and
  *                        near-to-far order (the reference culls octree nodes,
  *                        src/game.c:319-395)
@@ -242,6 +246,67 @@ int vp_gpu_splat_blocks(vp_ctx *ctx, const int16_t *d_splats, const uint8_t *d_l
 int vp_gpu_splat_backward(vp_ctx *ctx, uint64_t n, const vp_camera *cam,
                           const uint32_t *d_id, const float *d_weight,
                           const float *d_dL_drgb, float *d_grad, void *stream);
+
+/* ---- draw lists -----------------------------------------------------------
+ * The reference draws splats per octree node: at LOD level l a node covers
+ * the chunks whose coordinates shifted right by l equal the node's, and its
+ * vertex buffer is the concatenation, over those chunks in the order x
+ * outer, y, z inner, of each chunk's level-l records (gfx_update_svl,
+ * src/gfx/vsplat.c:197-338).  Nodes per axis at level l:
+ * 1 << (max_bitw[a] - min(l, max_bitw[a])); node index = flatten3 of the
+ * node coordinates with those bits (vsplat.c:213-232). */
+static inline uint32_t vp_node_bits(const vp_world_desc *w, uint32_t level, uint32_t axis)
+{
+    const uint32_t b = w->max_bitw[axis];
+    return b - (level < b ? level : b);
+}
+static inline uint64_t vp_node_count(const vp_world_desc *w, uint32_t level)
+{
+    return (uint64_t)1 << (vp_node_bits(w, level, 0) + vp_node_bits(w, level, 1) +
+                           vp_node_bits(w, level, 2));
+}
+
+typedef struct {
+    void     *d_node_splats;      /* capacity_records * 8 bytes                      */
+    uint64_t  capacity_records;
+    uint64_t *d_node_offsets;     /* vp_node_count + 1 exclusive prefix, in records  */
+} vp_node_out;
+
+/* Node lists of one level from a remesh output: d_splats, d_chunk_offsets
+ * (n+1, int16 items) and d_level_counts (n*5, int16 items) as
+ * vp_gpu_cull_mesh wrote them for the list d_chunk_ids[0..n).  Chunks not in
+ * the list contribute nothing; a chunk listed twice uses its last entry.
+ * d_node_splats = NULL with capacity 0 writes the offsets only.  Nodes that
+ * do not fit are not written (their offsets are) and VP_STATUS_OVERFLOW is
+ * raised.  Replaces the per-node uploads of gfx_update_svl. */
+int vp_gpu_svl_nodes(vp_ctx *ctx, const vp_world_desc *world, uint32_t level,
+                     const void *d_splats, const uint32_t *d_chunk_ids, uint32_t n,
+                     const uint64_t *d_chunk_offsets, const uint32_t *d_level_counts,
+                     const vp_node_out *out, void *stream);
+
+/* One draw of a frame: node `node` of level `level`, whose records go to
+ * the assembled list from record `first` on (vp_host_octree_cull fills it). */
+typedef struct {
+    uint32_t level;
+    uint32_t node;
+    uint64_t first;
+} vp_draw;
+
+/* Assemble a frame's splat list from its draws (device array d_draws[0..n)):
+ * each draw's node records are copied to d_out + 8 * first and its level to
+ * d_out_lod (log2 splat size per splat, the per-draw u_lod of
+ * vsplat.c:163-183).  d_node_splats / d_node_offsets: the five levels' node
+ * lists (host arrays of device pointers).  A draw naming a level or node
+ * outside the world raises VP_STATUS_BAD_ID, one reaching past
+ * capacity_records VP_STATUS_OVERFLOW; either is skipped.  The list is then one
+ * vp_gpu_splat(d_lod = d_out_lod) call: splat ids follow the draw order, so
+ * equal depths resolve to the earlier draw, as GL_LESS does for the
+ * reference's draw queue. */
+int vp_gpu_draw_gather(vp_ctx *ctx, const vp_world_desc *world,
+                       const void *const d_node_splats[5],
+                       const uint64_t *const d_node_offsets[5], const vp_draw *d_draws,
+                       uint32_t n, void *d_out, uint8_t *d_out_lod,
+                       uint64_t capacity_records, void *stream);
 
 #ifdef __cplusplus
 }

@@ -59,6 +59,29 @@ int vp_host_autotune_remesh(vp_ctx *ctx, const vp_world_desc *world,
                             void *stream, int *best_block, int *best_wg_per_cu,
                             float *best_ms);
 
+/* ---- frame draw queue ------------------------------------------------------
+ * Frustum planes of a view-projection matrix (column-major), normalised, in
+ * the order -x, +x, -y, +y, -z, +z of the clip-space bounds
+ * (gfx_fcull_extract_planes, src/gfx/fcull.c:18-36). */
+void vp_host_frustum_planes(const float view_projection[16], float planes[24]);
+
+/* The reference's per-frame LOD selection (octree_cull, src/game.c:319-395,
+ * driven from src/game.c:507-515): from every level-4 node, a node whose
+ * bounding sphere (radius 0.85 of its edge) passes the frustum test is drawn
+ * at its level if its centre is farther than level * 1024 from `origin`,
+ * else its eight children are visited (x outer, y, z inner).  At level 0 a
+ * chunk with chunk_mesh[chunk] != 0 (nullable) is queued as a mesh draw
+ * instead.  A node with no records is skipped, as a node without a vertex
+ * buffer is.  node_offsets[l]: host copy of level l's node offsets (from
+ * vp_gpu_svl_nodes).  Draws are written in queue order with `first` set to
+ * the running record total; returns VP_ERR_CAPACITY if draw_cap or mesh_cap
+ * is too small (the counts are still the full ones). */
+int vp_host_octree_cull(const vp_world_desc *world, const float planes[24], const float origin[3],
+                        const uint64_t *const node_offsets[5], const uint8_t *chunk_mesh,
+                        vp_draw *draws, uint32_t draw_cap, uint32_t *n_draws,
+                        uint32_t *mesh_chunks, uint32_t mesh_cap, uint32_t *n_mesh,
+                        uint64_t *total_records);
+
 #ifdef __cplusplus
 }
 #endif

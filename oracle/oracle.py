@@ -254,3 +254,120 @@ def splat_backward(n: int, cam: Camera, ids: np.ndarray, weight: np.ndarray,
                              _p(np.ascontiguousarray(weight, np.float32)),
                              _p(np.ascontiguousarray(dL, np.float32)), _p(g))
     return g
+
+
+# ---- draw lists (pure Python: small host-side walks) ------------------------
+def node_bits(w: WorldDesc, level: int):
+    """Node index bits per axis at a LOD level (vsplat.c:213-216)."""
+    return [w.max_bitw[a] - min(level, w.max_bitw[a]) for a in range(3)]
+
+
+def svl_nodes(w: WorldDesc, level: int, recs: np.ndarray, ids, offsets, counts):
+    """gfx_update_svl's node buffers for one level (vsplat.c:209-305).
+
+    recs / offsets (n+1, int16 items) / counts (n x 5, int16 items) are a
+    remesh output for the list `ids`; the last entry naming a chunk is its
+    svl.  Node (nx, ny, nz) concatenates, over chunks x in [nx<<l, (nx+1)<<l)
+    (outer), then y, then z (inner), clipped to the world, each chunk's
+    level-l records.  Returns (records int16[N, 4], node offsets u64[nodes+1])."""
+    recs = np.asarray(recs).reshape(-1, 4)
+    offsets = np.asarray(offsets, np.int64)
+    counts = np.asarray(counts, np.int64).reshape(-1, 5)
+    entry = {}
+    for k, c in enumerate(np.asarray(ids, np.int64)):
+        entry[int(c)] = k
+    nb = node_bits(w, level)
+    mx = [1 << b for b in w.max_bitw]
+    parts, offs = [], [0]
+    for nz in range(1 << nb[2]):
+        for ny in range(1 << nb[1]):
+            for nx in range(1 << nb[0]):
+                lo = [nx << level, ny << level, nz << level]
+                hi = [min((v + 1) << level, m) for v, m in zip((nx, ny, nz), mx)]
+                total = 0
+                for x in range(lo[0], hi[0]):
+                    for y in range(lo[1], hi[1]):
+                        for z in range(lo[2], hi[2]):
+                            c = ((z << w.max_bitw[1] | y) << w.max_bitw[0]) | x
+                            k = entry.get(c)
+                            if k is None or counts[k, level] == 0:
+                                continue
+                            start = (offsets[k] + counts[k, :level].sum()) // 4
+                            parts.append(recs[start:start + counts[k, level] // 4])
+                            total += counts[k, level] // 4
+                offs.append(offs[-1] + total)
+    out = np.concatenate(parts) if parts else np.zeros((0, 4), np.int16)
+    return out, np.asarray(offs, np.uint64)
+
+
+def frustum_planes(m16) -> np.ndarray:
+    """gfx_fcull_extract_planes (fcull.c:18-36) in float32 arithmetic."""
+    m = np.asarray(m16, np.float32)
+    f = np.float32
+    p = []
+    for i in range(3):
+        p += [f(m[3 + 4 * j] - m[4 * j + i]) for j in range(4)]
+        p += [f(m[3 + 4 * j] + m[4 * j + i]) for j in range(4)]
+    out = np.empty(24, np.float32)
+    for i in range(6):
+        a, b, c, d = p[4 * i:4 * i + 4]
+        sq = f(f(f(a * a) + f(b * b)) + f(c * c))
+        ln = f(np.sqrt(np.float64(sq)))
+        out[4 * i:4 * i + 4] = [f(a / ln), f(b / ln), f(c / ln), f(d / ln)]
+    return out
+
+
+def octree_cull(w: WorldDesc, planes, origin, node_items, chunk_mesh=None):
+    """octree_cull (game.c:319-395) from every level-4 node (game.c:507-512).
+
+    node_items[l][node]: records of node `node` at level l (0 = no buffer).
+    Returns (draws [(level, node, first)], mesh chunks, total records)."""
+    f = np.float32
+    planes = np.asarray(planes, np.float32)
+    o = [f(v) for v in origin]
+    R = 1 << w.root_bitw
+    mx = [1 << b for b in w.max_bitw]
+    draws, mesh = [], []
+    total = [0]
+
+    def visible(c, radius):
+        for i in range(6):
+            p = planes[4 * i:4 * i + 4]
+            dot = f(f(f(p[0] * c[0]) + f(p[1] * c[1])) + f(p[2] * c[2]))
+            if f(dot + p[3]) < f(-radius):
+                return False
+        return True
+
+    def walk(lod, x, y, z):
+        ws = R << lod
+        if (x << lod) >= mx[0] or (y << lod) >= mx[1] or (z << lod) >= mx[2]:
+            return
+        c = [f(v * ws + ws // 2) for v in (x, y, z)]
+        if not visible(c, f(ws * 0.85)):
+            return
+        d = [f(o[i] - c[i]) for i in range(3)]
+        dist = f(np.sqrt(f(f(f(d[0] * d[0]) + f(d[1] * d[1])) + f(d[2] * d[2]))))
+        if dist > f(lod * 1024):
+            if lod == 0 and chunk_mesh is not None:
+                chunk = ((z << w.max_bitw[1] | y) << w.max_bitw[0]) | x
+                if chunk_mesh[chunk]:
+                    mesh.append(chunk)
+                    return
+            b = node_bits(w, lod)
+            node = ((z << b[1] | y) << b[0]) | x
+            items = int(node_items[lod][node])
+            if items == 0:
+                return
+            draws.append((lod, node, total[0]))
+            total[0] += items
+            return
+        for nx in range(2):
+            for ny in range(2):
+                for nz in range(2):
+                    walk(lod - 1, 2 * x + nx, 2 * y + ny, 2 * z + nz)
+
+    for x in range((mx[0] >> 4) + 1):
+        for y in range((mx[1] >> 4) + 1):
+            for z in range((mx[2] >> 4) + 1):
+                walk(4, x, y, z)
+    return draws, mesh, total[0]

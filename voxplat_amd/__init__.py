@@ -5,6 +5,8 @@ include/vp_gpu.h.
     from voxplat_amd import GPU, abi, workload
 """
 from . import abi, workload
-from ._lib import GPU, RemeshGraph, VPError, host_shadow_spheres, load_library
+from ._lib import (GPU, RemeshGraph, VPError, frustum_planes, host_shadow_spheres, load_library,
+                   octree_cull)
 
-__all__ = ["GPU", "RemeshGraph", "VPError", "abi", "workload", "load_library", "host_shadow_spheres"]
+__all__ = ["GPU", "RemeshGraph", "VPError", "abi", "workload", "load_library", "host_shadow_spheres",
+           "frustum_planes", "octree_cull"]

@@ -293,6 +293,61 @@ class GPU:
         return grad
 
     # ---- workload builders (include/vp_workload.h) -------------------------------
+    # ---- draw lists --------------------------------------------------------------
+    def svl_nodes(self, world: abi.WorldDesc, level: int, splats, ids, offsets, counts,
+                  capacity: int | None = None, out=None, stream=None):
+        """Node lists of one level from a remesh output.  Returns (records
+        int16[N,4] or None, node offsets int64[nodes+1] in records)."""
+        torch = self.torch
+        n = int(ids.numel())
+        dev = ids.device
+        nn = abi.node_count(world, level)
+        if out is None:
+            recs, noffs = None, torch.empty(nn + 1, dtype=torch.int64, device=dev)
+        else:
+            recs, noffs = out
+        st = C.c_void_p(self._stream(stream))
+        no = abi.NodeOut()
+        no.d_node_offsets = noffs.data_ptr()
+
+        def run(what):
+            _check(self.lib.vp_gpu_svl_nodes(
+                self.ctx, C.byref(world), level, C.c_void_p(_ptr(splats)), C.c_void_p(_ptr(ids)), n,
+                C.c_void_p(_ptr(offsets)), C.c_void_p(_ptr(counts)), C.byref(no), st), what)
+        if recs is None and capacity is None:
+            no.d_node_splats, no.capacity_records = None, 0
+            run("vp_gpu_svl_nodes(count)")
+            capacity = int(noffs[-1].item())
+        if recs is None:
+            recs = torch.empty((max(capacity, 1), 4), dtype=torch.int16, device=dev)
+        elif capacity is None:
+            capacity = int(recs.shape[0])
+        no.d_node_splats, no.capacity_records = recs.data_ptr(), capacity
+        run("vp_gpu_svl_nodes")
+        return recs, noffs
+
+    def draw_gather(self, world: abi.WorldDesc, nodes, node_offsets, draws, total: int,
+                    out=None, stream=None):
+        """Assemble a frame's list from its draws (structured array from
+        octree_cull).  Returns (records int16[total,4], lod uint8[total])."""
+        import numpy as np
+        torch = self.torch
+        dev = node_offsets[0].device
+        if out is None:
+            recs = torch.empty((max(total, 1), 4), dtype=torch.int16, device=dev)
+            lod = torch.empty(max(total, 1), dtype=torch.uint8, device=dev)
+        else:
+            recs, lod = out
+        d = torch.from_numpy(np.ascontiguousarray(draws).view(np.uint8).copy()).to(dev)
+        nptr = (C.c_void_p * 5)(*[t.data_ptr() for t in nodes])
+        optr = (C.c_void_p * 5)(*[t.data_ptr() for t in node_offsets])
+        _check(self.lib.vp_gpu_draw_gather(self.ctx, C.byref(world), nptr, optr,
+                                           C.c_void_p(d.data_ptr()), len(draws),
+                                           C.c_void_p(recs.data_ptr()), C.c_void_p(lod.data_ptr()),
+                                           int(recs.shape[0]), C.c_void_p(self._stream(stream))),
+               "vp_gpu_draw_gather")
+        return recs, lod
+
     def generate(self, world: abi.WorldDesc, params: abi.GenParams, voxels, uniform=None,
                  chunk0: int = 0, count: int | None = None, stream=None):
         count = world.chunks - chunk0 if count is None else count
@@ -323,6 +378,50 @@ class GPU:
                                             C.c_void_p(_ptr(dirty)), C.c_void_p(_ptr(uniform)),
                                             C.c_void_p(self._stream(stream))),
                "vp_gpu_edit_spheres")
+
+
+def _draw_dtype():
+    import numpy as np
+    return np.dtype([("level", "<u4"), ("node", "<u4"), ("first", "<u8")])
+
+
+def frustum_planes(view_projection):
+    """vp_host_frustum_planes: 24 floats (6 normalised planes) of a
+    column-major view-projection matrix."""
+    import numpy as np
+    m = np.ascontiguousarray(view_projection, np.float32)
+    out = np.empty(24, np.float32)
+    load_library().vp_host_frustum_planes(m.ctypes.data_as(C.POINTER(C.c_float)),
+                                          out.ctypes.data_as(C.POINTER(C.c_float)))
+    return out
+
+
+def octree_cull(world: abi.WorldDesc, planes, origin, node_offsets, chunk_mesh=None):
+    """vp_host_octree_cull.  node_offsets: five host uint64 arrays (the
+    levels' node offsets).  Returns (draws: structured array of
+    (level, node, first), mesh chunk ids uint32, total records)."""
+    import numpy as np
+    lib = load_library()
+    planes = np.ascontiguousarray(planes, np.float32)
+    origin = np.ascontiguousarray(origin, np.float32)
+    offs = [np.ascontiguousarray(o, np.uint64) for o in node_offsets]
+    optr = (C.c_void_p * 5)(*[o.ctypes.data for o in offs])
+    cm = None if chunk_mesh is None else np.ascontiguousarray(chunk_mesh, np.uint8)
+    nd, nm, tot = C.c_uint32(), C.c_uint32(), C.c_uint64()
+    cap_d, cap_m = 256, 256
+    while True:
+        draws = np.zeros(cap_d, _draw_dtype())
+        mesh = np.zeros(cap_m, np.uint32)
+        rc = lib.vp_host_octree_cull(C.byref(world), planes.ctypes.data_as(C.POINTER(C.c_float)),
+                                     origin.ctypes.data_as(C.POINTER(C.c_float)), optr,
+                                     None if cm is None else cm.ctypes.data,
+                                     draws.ctypes.data_as(C.POINTER(abi.Draw)), cap_d, C.byref(nd),
+                                     mesh.ctypes.data, cap_m, C.byref(nm), C.byref(tot))
+        if rc == abi.VP_ERR_CAPACITY:
+            cap_d, cap_m = max(cap_d, nd.value), max(cap_m, nm.value)
+            continue
+        _check(rc, "vp_host_octree_cull")
+        return draws[:nd.value].copy(), mesh[:nm.value].copy(), tot.value
 
 
 def host_shadow_spheres(world: abi.WorldDesc, smap_host, spheres) -> None:

@@ -70,6 +70,25 @@ class MeshOut(C.Structure):
                 ("capacity_faces", C.c_uint64), ("d_face_offsets", C.c_void_p)]
 
 
+class NodeOut(C.Structure):
+    _fields_ = [("d_node_splats", C.c_void_p), ("capacity_records", C.c_uint64),
+                ("d_node_offsets", C.c_void_p)]
+
+
+class Draw(C.Structure):
+    """vp_draw: one node draw of a frame."""
+    _fields_ = [("level", C.c_uint32), ("node", C.c_uint32), ("first", C.c_uint64)]
+
+
+def node_bits(w: "WorldDesc", level: int, axis: int) -> int:
+    b = w.max_bitw[axis]
+    return b - min(level, b)
+
+
+def node_count(w: "WorldDesc", level: int) -> int:
+    return 1 << (node_bits(w, level, 0) + node_bits(w, level, 1) + node_bits(w, level, 2))
+
+
 class Camera(C.Structure):
     """vp_camera: column-major matrices as uploaded by gfx_svl_draw (vsplat.c:126-158)."""
     _fields_ = [("view_translation", C.c_float * 16), ("proj_view_rot", C.c_float * 16),
@@ -111,6 +130,12 @@ PROTOTYPES = {
     "vp_gpu_splat_blocks": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint64,
                                       C.c_void_p, C.POINTER(Camera), C.c_void_p, C.POINTER(Frame),
                                       C.c_void_p]),
+    "vp_gpu_svl_nodes": (C.c_int, [C.c_void_p, C.POINTER(WorldDesc), C.c_uint32, C.c_void_p,
+                                   C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p,
+                                   C.POINTER(NodeOut), C.c_void_p]),
+    "vp_gpu_draw_gather": (C.c_int, [C.c_void_p, C.POINTER(WorldDesc), C.c_void_p * 5,
+                                     C.c_void_p * 5, C.c_void_p, C.c_uint32, C.c_void_p,
+                                     C.c_void_p, C.c_uint64, C.c_void_p]),
 }
 VP_SPLAT_BLOCK = 256
 
@@ -160,4 +185,10 @@ PROTOTYPES.update({
                                           C.c_uint32, C.c_uint32, C.POINTER(SplatOut), C.c_int,
                                           C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int),
                                           C.POINTER(C.c_float)]),
+    "vp_host_frustum_planes": (None, [C.POINTER(C.c_float), C.POINTER(C.c_float)]),
+    "vp_host_octree_cull": (C.c_int, [C.POINTER(WorldDesc), C.POINTER(C.c_float),
+                                      C.POINTER(C.c_float), C.c_void_p * 5, C.c_void_p,
+                                      C.POINTER(Draw), C.c_uint32, C.POINTER(C.c_uint32),
+                                      C.c_void_p, C.c_uint32, C.POINTER(C.c_uint32),
+                                      C.POINTER(C.c_uint64)]),
 })

@@ -34,7 +34,7 @@ HIP_FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     f"-I{ROOT / 'include'}",
 ]
-C_FLAGS = ["-O2", "-fPIC", "-std=c11", "-Wall", "-Wextra",
+C_FLAGS = ["-O2", "-fPIC", "-std=c11", "-Wall", "-Wextra", "-ffp-contract=off",
            "-D__HIP_PLATFORM_AMD__", f"-I{ROCM}/include", f"-I{ROOT / 'include'}"]
 
 

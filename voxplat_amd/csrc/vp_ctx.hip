@@ -58,6 +58,20 @@ int ensure_blocks(vp_ctx *ctx, uint64_t blocks)
     return VP_OK;
 }
 
+int ensure_cmap(vp_ctx *ctx, uint64_t chunks)
+{
+    if (chunks <= ctx->cmap_cap) return VP_OK;
+    uint32_t *p = nullptr;
+    if (hipMalloc(&p, chunks * sizeof(uint32_t)) != hipSuccess) return VP_ERR_NOMEM;
+    if (ctx->d_cmap) {
+        if (hipDeviceSynchronize() != hipSuccess) return VP_ERR_HIP;
+        (void)hipFree(ctx->d_cmap);
+    }
+    ctx->d_cmap = p;
+    ctx->cmap_cap = chunks;
+    return VP_OK;
+}
+
 }  // namespace vp
 
 extern "C" int vp_ctx_create(int device, vp_ctx **out)
@@ -100,6 +114,7 @@ extern "C" int vp_ctx_destroy(vp_ctx *ctx)
     if (ctx->d_frame) (void)hipFree(ctx->d_frame);
     if (ctx->d_rays) (void)hipFree(ctx->d_rays);
     if (ctx->d_blk) (void)hipFree(ctx->d_blk);
+    if (ctx->d_cmap) (void)hipFree(ctx->d_cmap);
     delete ctx;
     return VP_OK;
 }

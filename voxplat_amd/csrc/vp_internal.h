@@ -30,6 +30,8 @@ struct vp_ctx {
     uint32_t *d_blk = nullptr;         // splat block culling: bins, order, counters
     uint64_t  blk_cap = 0;             // blocks d_blk is sized for
     uint64_t  rays_cap = 0;
+    uint32_t *d_cmap = nullptr;        // draw lists: list entry + 1 per chunk index
+    uint64_t  cmap_cap = 0;
     // tuned launch shape of the persistent remesh kernel
     int cull_block = 512;              // threads per workgroup
     int cull_wg_per_cu = 2;            // resident workgroups per CU
@@ -40,6 +42,7 @@ namespace vp {
 int ensure_state(vp_ctx *ctx, uint64_t words);
 int ensure_frame(vp_ctx *ctx, uint64_t pixels);
 int ensure_blocks(vp_ctx *ctx, uint64_t blocks);
+int ensure_cmap(vp_ctx *ctx, uint64_t chunks);
 
 inline hipStream_t as_stream(void *s) { return reinterpret_cast<hipStream_t>(s); }
 

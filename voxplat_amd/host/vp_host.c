@@ -1,7 +1,8 @@
 /*
  * vp_host.c -- host-side C of the pipeline: the order-dependent shadow-map
- * updates of sphere edits, and the remesh drivers (plain per-chunk loop,
- * captured batched frame, launch-shape autotuner) declared in vp_host.h.
+ * updates of sphere edits, the remesh drivers (plain per-chunk loop,
+ * captured batched frame, launch-shape autotuner) and the per-frame draw
+ * queue (frustum planes, octree LOD selection) declared in vp_host.h.
  *
  * shadow_place_update (src/chunkset/shadow.h:77-89) reads map[i] and map[i+1]
  * before writing map[i], so the result depends on the order of the placed
@@ -11,6 +12,7 @@
  * (x outer, then y, z).  The updates are cheap and strictly sequential, so
  * they run here, on a host copy of the map.
  */
+#include <math.h>
 #include <stdint.h>
 #include <stddef.h>
 
@@ -211,4 +213,118 @@ int vp_host_autotune_remesh(vp_ctx *ctx, const vp_world_desc *world,
     *best_wg_per_cu = bw;
     if (best_ms) *best_ms = best;
     return vp_ctx_set_launch(ctx, bb, bw);
+}
+
+/* ---- frame draw queue -------------------------------------------------------
+ * gfx_fcull_extract_planes (src/gfx/fcull.c:18-36): rows w - x, w + x,
+ * w - y, w + y, w - z, w + z of the column-major view-projection, each
+ * divided by the length of its xyz part (a float sum of squares, square root
+ * in double, as vec3length computes it). */
+void vp_host_frustum_planes(const float m[16], float planes[24])
+{
+    int n = 0;
+    for (int i = 0; i < 3; i++) {
+        for (int j = 0; j < 4; j++) planes[n++] = m[3 + 4 * j] - m[4 * j + i];
+        for (int j = 0; j < 4; j++) planes[n++] = m[3 + 4 * j] + m[4 * j + i];
+    }
+    for (int i = 0; i < 6; i++) {
+        float *p = planes + 4 * i;
+        const float sq = p[0] * p[0] + p[1] * p[1] + p[2] * p[2];
+        const float len = (float)sqrt((double)sq);
+        p[0] /= len;
+        p[1] /= len;
+        p[2] /= len;
+        p[3] /= len;
+    }
+}
+
+typedef struct {
+    const vp_world_desc *w;
+    const float *planes;
+    const float *origin;
+    const uint64_t *const *node_off;
+    const uint8_t *chunk_mesh;
+    vp_draw *draws;
+    uint32_t draw_cap, n_draws;
+    uint32_t *mesh;
+    uint32_t mesh_cap, n_mesh;
+    uint64_t total;
+} cull_state;
+
+/* gfx_fcull_visible (fcull.c:38-45): outside iff some plane's signed
+ * distance is below -radius */
+static int sphere_visible(const float *planes, const float c[3], float radius)
+{
+    for (int i = 0; i < 6; i++) {
+        const float *p = planes + 4 * i;
+        const float d = (p[0] * c[0] + p[1] * c[1]) + p[2] * c[2];
+        if (d + p[3] < -radius) return 0;
+    }
+    return 1;
+}
+
+/* octree_cull (src/game.c:319-395) */
+static void cull_node(cull_state *s, uint32_t lod, uint32_t x, uint32_t y, uint32_t z)
+{
+    const vp_world_desc *w = s->w;
+    const uint32_t ws_root = (uint32_t)(1u << w->root_bitw) << lod;
+    if ((x << lod) >= (1u << w->max_bitw[0]) || (y << lod) >= (1u << w->max_bitw[1]) ||
+        (z << lod) >= (1u << w->max_bitw[2]))
+        return;
+    const float c[3] = {(float)(x * ws_root + ws_root / 2), (float)(y * ws_root + ws_root / 2),
+                        (float)(z * ws_root + ws_root / 2)};
+    if (!sphere_visible(s->planes, c, (float)((double)ws_root * 0.85))) return;
+    const float d0 = s->origin[0] - c[0], d1 = s->origin[1] - c[1], d2 = s->origin[2] - c[2];
+    const float dist = sqrtf((d0 * d0 + d1 * d1) + d2 * d2);
+    if (dist > (float)(lod * 1024u)) {
+        if (lod == 0 && s->chunk_mesh) {
+            const uint32_t chunk = ((z << w->max_bitw[1] | y) << w->max_bitw[0]) | x;
+            if (s->chunk_mesh[chunk]) {
+                if (s->n_mesh < s->mesh_cap) s->mesh[s->n_mesh] = chunk;
+                s->n_mesh++;
+                return;
+            }
+        }
+        const uint32_t b0 = vp_node_bits(w, lod, 0), b1 = vp_node_bits(w, lod, 1);
+        const uint32_t node = ((z << b1 | y) << b0) | x;
+        const uint64_t items = s->node_off[lod][node + 1] - s->node_off[lod][node];
+        if (!items) return;                    /* no vertex buffer */
+        if (s->n_draws < s->draw_cap) {
+            vp_draw *d = s->draws + s->n_draws;
+            d->level = lod;
+            d->node = node;
+            d->first = s->total;
+        }
+        s->n_draws++;
+        s->total += items;
+        return;
+    }
+    for (uint32_t nx = 0; nx < 2; nx++)
+        for (uint32_t ny = 0; ny < 2; ny++)
+            for (uint32_t nz = 0; nz < 2; nz++)
+                cull_node(s, lod - 1, 2 * x + nx, 2 * y + ny, 2 * z + nz);
+}
+
+int vp_host_octree_cull(const vp_world_desc *world, const float planes[24], const float origin[3],
+                        const uint64_t *const node_offsets[5], const uint8_t *chunk_mesh,
+                        vp_draw *draws, uint32_t draw_cap, uint32_t *n_draws,
+                        uint32_t *mesh_chunks, uint32_t mesh_cap, uint32_t *n_mesh,
+                        uint64_t *total_records)
+{
+    if (!world || !planes || !origin || !node_offsets || !n_draws || !n_mesh) return VP_ERR_ARG;
+    if ((draw_cap && !draws) || (mesh_cap && !mesh_chunks)) return VP_ERR_ARG;
+    for (int l = 0; l < 5; l++)
+        if (!node_offsets[l]) return VP_ERR_ARG;
+    cull_state s = {world, planes, origin, node_offsets, chunk_mesh, draws, draw_cap, 0,
+                    mesh_chunks, mesh_cap, 0, 0};
+    /* every level-4 node, x outer, y, z inner (game.c:507-512) */
+    const uint32_t mx = 1u << world->max_bitw[0], my = 1u << world->max_bitw[1],
+                   mz = 1u << world->max_bitw[2];
+    for (uint32_t x = 0; x < (mx >> 4) + 1; x++)
+        for (uint32_t y = 0; y < (my >> 4) + 1; y++)
+            for (uint32_t z = 0; z < (mz >> 4) + 1; z++) cull_node(&s, 4, x, y, z);
+    *n_draws = s.n_draws;
+    *n_mesh = s.n_mesh;
+    if (total_records) *total_records = s.total;
+    return (s.n_draws > draw_cap || s.n_mesh > mesh_cap) ? VP_ERR_CAPACITY : VP_OK;
 }

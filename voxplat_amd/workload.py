@@ -108,6 +108,17 @@ def make_camera(location, yaw_deg: float, pitch_deg: float, width: int, height: 
     return cam
 
 
+def make_view_projection(location, yaw_deg: float, pitch_deg: float, width: int, height: int,
+                         fov_deg: float = 70.0, far: float = 4096.0, near: float = 0.1) -> np.ndarray:
+    """projection * view_rotation * view_translation of make_camera's pose,
+    column-major float32[16] (game.c:470-483); its frustum planes and the
+    camera location drive the octree LOD selection."""
+    P = _perspective(math.radians(fov_deg), width / float(height), near, far)
+    T = _translation([-location[0], -location[1], -location[2]])
+    Rv = _rotation(math.radians(pitch_deg), (1, 0, 0)) @ _rotation(math.radians(yaw_deg), (0, 1, 0))
+    return np.asarray(_colmajor(P @ (Rv @ T)), np.float32)
+
+
 def splat_poses(terrain_height, n: int = 16, seed: int = SPLAT_SEED, lo: float = 256.0,
                 hi: float = 1792.0):
     """Seeded poses: x, z in [lo, hi], 64..512 above the terrain, yaw in
