@@ -10,6 +10,8 @@ Also measured (not the headline value):
   * the plain per-chunk path: one launch per chunk from the C host loop;
   * splat resolve of the level-0 list at 1920x1080 over 16 seeded poses;
   * the near-field quad mesh (chunk_make_mesh) of every chunk of the world;
+  * the octree draw lists: node buffers of all five levels, then per pose the
+    LOD selection, the assembled list and its splat resolve at 1920x1080;
   * the roofline of the remesh kernel from HIP events around the timed steps;
   * the CPU oracle (4 OpenMP threads, as chunkset_manage's team) on the same
     4096 chunks.
@@ -270,6 +272,69 @@ def main() -> int:
         log(f"splat: {n0} splats, {fr_ms:.2f} ms/frame -> {n0 / (fr_ms * 1e-3) / 1e9:.2f} Gsplats/s")
         splat_cpu = (l0[::32].contiguous().cpu().numpy(), cams[0])   # every 32nd splat: ~4 M
 
+    # ---- draw lists: octree node buffers, then per pose the LOD selection,
+    # the assembled list and its splat resolve at 1080p ----
+    draw_res = None
+    if not args.no_splat:
+        from voxplat_amd import frustum_planes, octree_cull
+
+        for _ in range(2):
+            nodes = [gpu.svl_nodes(w, l, splats, ids, offs, counts, capacity=None) for l in range(5)]
+        torch.cuda.synchronize()
+        caps = [int(n[1][-1].item()) for n in nodes]
+        lev_ev = [torch.cuda.Event(enable_timing=True) for _ in range(6)]
+        lev_ev[0].record(stream)
+        for l in range(5):
+            gpu.svl_nodes(w, l, splats, ids, offs, counts, capacity=caps[l], out=nodes[l])
+            lev_ev[l + 1].record(stream)
+        torch.cuda.synchronize()
+        assert gpu.status() == 0
+        n_ms = lev_ev[0].elapsed_time(lev_ev[5])
+        lev_ms = [round(lev_ev[l].elapsed_time(lev_ev[l + 1]), 4) for l in range(5)]
+        node_bytes = 16 * sum(caps)                          # every record read once, written once
+        node_offs_h = [n[1].cpu().numpy().view(np.uint64) for n in nodes]
+        lst = torch.empty((caps[0] + caps[1], 4), dtype=torch.int16, device=dev)
+        lodb = torch.empty(caps[0] + caps[1], dtype=torch.uint8, device=dev)
+        bnd_buf = torch.empty(((caps[0] + caps[1]) // abi.VP_SPLAT_BLOCK + 1, 8), dtype=torch.float32,
+                              device=dev)
+        depth_d = torch.empty((1080, 1920), dtype=torch.float32, device=dev)
+        id_d = torch.empty((1080, 1920), dtype=torch.int32, device=dev)
+        cull_s, gather_ms, frame_ms, drawn = [], [], [], []
+        for p, yw, pt in poses:
+            t0 = time.perf_counter()
+            planes = frustum_planes(W.make_view_projection(p, yw, pt, 1920, 1080))
+            draws, _, total = octree_cull(w, planes, p, node_offs_h)
+            cull_s.append(time.perf_counter() - t0)
+            if total == 0:
+                continue
+            cam = W.make_camera(p, yw, pt, 1920, 1080)
+            torch.cuda.synchronize()
+            e0.record(stream)
+            gpu.draw_gather(w, [n[0] for n in nodes], [n[1] for n in nodes], draws, total,
+                            out=(lst, lodb))
+            e2 = torch.cuda.Event(enable_timing=True)
+            e2.record(stream)
+            # the assembled list changes every frame: its block bounds too
+            bnd = gpu.splat_bounds(lst[:total], lod=lodb[:total], out=bnd_buf)
+            gpu.splat(lst[:total], cam, lod=lodb[:total], out=(depth_d, id_d, None, None), bounds=bnd)
+            e1.record(stream)
+            torch.cuda.synchronize()
+            gather_ms.append(e0.elapsed_time(e2))   # gather; the frame adds bounds + resolve
+            frame_ms.append(e0.elapsed_time(e1))
+            drawn.append(total)
+        assert gpu.status() == 0
+        draw_res = {"node_buffers_ms": round(n_ms, 4), "node_ms_per_level": lev_ms,
+                    "node_records": sum(caps),
+                    "node_gb_s": round(node_bytes / (n_ms * 1e-3) / 1e9, 1),
+                    "poses": len(frame_ms), "resolution": "1920x1080",
+                    "mean_splats_drawn": int(np.mean(drawn)) if drawn else 0,
+                    "host_lod_selection_ms": round(1e3 * float(np.mean(cull_s)), 3),
+                    "gather_ms": round(float(np.mean(gather_ms)), 4) if gather_ms else None,
+                    "frame_ms": round(float(np.mean(frame_ms)), 4) if frame_ms else None}
+        log(f"draw lists: nodes {n_ms:.3f} ms; per frame {draw_res['mean_splats_drawn']} splats, "
+            f"{draw_res['frame_ms']} ms (gather {draw_res['gather_ms']} ms)")
+        del nodes, lst, lodb, bnd_buf
+
     # ---- near-field quad mesh of the same world (every chunk, index order) ----
     mesh_res = None
     if not args.no_mesh:
@@ -399,6 +464,7 @@ def main() -> int:
         "speedup_vs_per_chunk": round(gvox / pc_gvox, 2),
         "splat": splat_res,
         "mesh": mesh_res,
+        "draw": draw_res,
         "cfg0": cfg0_res,
         "roofline": roofline,
         "cpu_baseline": cpu,

@@ -239,12 +239,17 @@ class GPU:
         return voxels
 
     # ---- splat -----------------------------------------------------------------
-    def splat_bounds(self, splats, lod=None, lod_log2: int = 0, stream=None):
-        """Per-block bounds (float32 [ceil(n/256), 8]) for splat(..., bounds=...)."""
+    def splat_bounds(self, splats, lod=None, lod_log2: int = 0, out=None, stream=None):
+        """Per-block bounds (float32 [ceil(n/256), 8]) for splat(..., bounds=...);
+        out: a buffer of at least that many rows to write them to."""
         torch = self.torch
         n = int(splats.shape[0])
         nb = (n + abi.VP_SPLAT_BLOCK - 1) // abi.VP_SPLAT_BLOCK
-        b = torch.empty((max(nb, 1), 8), dtype=torch.float32, device=splats.device)
+        if out is not None:
+            assert out.shape[0] >= nb and out.dtype == torch.float32
+            b = out[:max(nb, 1)]
+        else:
+            b = torch.empty((max(nb, 1), 8), dtype=torch.float32, device=splats.device)
         _check(self.lib.vp_gpu_splat_bounds(self.ctx, C.c_void_p(splats.data_ptr()),
                                             C.c_void_p(_ptr(lod)), lod_log2, n,
                                             C.c_void_p(b.data_ptr()),

@@ -22,7 +22,7 @@ namespace vp {
 constexpr uint64_t kNFlagAgg = 1ull << 62;
 constexpr uint64_t kNFlagPre = 2ull << 62;
 constexpr uint64_t kNValMask = (1ull << 62) - 1;
-constexpr int kNodeBS = 256;
+constexpr int kNodeBS = 512;
 constexpr int kNodeMaxSub = 4096;   // chunks of a level-4 node: 16^3
 
 struct NodeArgs {
@@ -195,15 +195,32 @@ __global__ void __launch_bounds__(kNodeBS) k_svl_nodes(NodeArgs a)
         __syncthreads();
         const uint64_t base = s_base;
         if (a.out && total && base + total <= a.cap) {
-            for (uint32_t j = threadIdx.x; j < total; j += kNodeBS) {
-                // the last chunk whose prefix is <= j holds record j
-                uint32_t lo = 0, hi = S - 1u;
-                while (lo < hi) {
-                    const uint32_t mid = (lo + hi + 1u) >> 1;
-                    if (pre[mid] <= j) lo = mid;
-                    else hi = mid - 1u;
+            // U records per thread per step, all loads issued before any
+            // store: a node of a high level is one of few workgroups, so its
+            // copy has to keep many loads in flight on its own
+            constexpr int U = 4;
+            for (uint32_t j0 = threadIdx.x; j0 < total; j0 += U * kNodeBS) {
+                uint64_t v[U];
+#pragma unroll
+                for (int u = 0; u < U; u++) {
+                    const uint32_t j = j0 + (uint32_t)u * kNodeBS;
+                    v[u] = 0;
+                    if (j < total) {
+                        // the last chunk whose prefix is <= j holds record j
+                        uint32_t lo = 0, hi = S - 1u;
+                        while (lo < hi) {
+                            const uint32_t mid = (lo + hi + 1u) >> 1;
+                            if (pre[mid] <= j) lo = mid;
+                            else hi = mid - 1u;
+                        }
+                        v[u] = a.splats[src[lo] + (j - pre[lo])];
+                    }
                 }
-                a.out[base + j] = a.splats[src[lo] + (j - pre[lo])];
+#pragma unroll
+                for (int u = 0; u < U; u++) {
+                    const uint32_t j = j0 + (uint32_t)u * kNodeBS;
+                    if (j < total) a.out[base + j] = v[u];
+                }
             }
         }
         __syncthreads();
@@ -237,9 +254,22 @@ __global__ void __launch_bounds__(256) k_draw_gather(GatherArgs a)
             continue;
         }
         const uint64_t *src = a.nodes[l] + s0;
-        for (uint64_t j = threadIdx.x; j < cnt; j += blockDim.x) {
-            a.out[first + j] = src[j];
-            a.lod[first + j] = (uint8_t)l;
+        constexpr int U = 4;                    // loads in flight per thread
+        for (uint64_t j0 = threadIdx.x; j0 < cnt; j0 += U * blockDim.x) {
+            uint64_t v[U];
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                const uint64_t j = j0 + (uint64_t)u * blockDim.x;
+                v[u] = j < cnt ? src[j] : 0ull;
+            }
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                const uint64_t j = j0 + (uint64_t)u * blockDim.x;
+                if (j < cnt) {
+                    a.out[first + j] = v[u];
+                    a.lod[first + j] = (uint8_t)l;
+                }
+            }
         }
     }
 }
