@@ -1,7 +1,7 @@
 """Plain launches of the batched remesh on a generated world, for profilers.
 
     python -m voxplat_amd.remesh_run [--world 2k|16k] [--launches K] [--copy-gib G]
-                                     [--kernel cull|mesh]
+                                     [--kernel cull|mesh|nodes]
 
 Generates the world on the GPU, orders the chunk list heaviest first and
 sizes the output with counting passes, then issues K direct (not
@@ -10,7 +10,9 @@ kernel tracer or counter collector sees one dispatch per launch.
 --copy-gib G first runs one device-to-device copy of G GiB: a kernel of
 known traffic (G GiB read, G GiB written) against which HBM byte counters
 are scaled.  --kernel mesh issues vp_gpu_mesh (the near-field quad mesh)
-over every chunk in index order instead, output sized by a counting call.
+over every chunk in index order instead, output sized by a counting call;
+--kernel nodes remeshes once and then issues vp_gpu_svl_nodes for the five
+LOD levels per launch (the octree node buffers).
 """
 from __future__ import annotations
 
@@ -32,7 +34,7 @@ def main(argv=None) -> int:
     ap.add_argument("--launches", type=int, default=3)
     ap.add_argument("--copy-gib", type=float, default=0.0)
     ap.add_argument("--autotune", action="store_true", help="pick the launch shape first")
-    ap.add_argument("--kernel", choices=("cull", "mesh"), default="cull")
+    ap.add_argument("--kernel", choices=("cull", "mesh", "nodes"), default="cull")
     a = ap.parse_args(argv)
     gpu = GPU(0)
     if a.copy_gib:
@@ -56,6 +58,8 @@ def main(argv=None) -> int:
         return run_mesh(gpu, cfg, w, vox, uni, smap, ids, a.launches)
     ids, offs, counts = heaviest_first(gpu, w, vox, ids, smap, uni)
     torch.cuda.synchronize()
+    if a.kernel == "nodes":
+        return run_nodes(gpu, cfg, w, vox, uni, smap, ids, offs, counts, a.launches)
     cap = int(offs[-1].item()) // 4
     splats = torch.empty((max(cap, 1), 4), dtype=torch.int16, device="cuda")
     if a.autotune:   # the launch shape the benchmark would pick
@@ -74,6 +78,29 @@ def main(argv=None) -> int:
     st = gpu.status()
     print(json.dumps({"world": cfg.name, "chunks": w.chunks, "records": cap,
                       "launch_ms": [round(x, 4) for x in ms], "status": st}))
+    gpu.close()
+    return 0 if st == 0 else 1
+
+
+def run_nodes(gpu, cfg, w, vox, uni, smap, ids, offs, counts, launches: int) -> int:
+    recs, _, _ = gpu.cull_mesh(w, vox, ids, shadow_map=smap, uniform=uni,
+                               capacity=int(offs[-1].item()) // 4, out=(offs, counts))
+    nodes = [gpu.svl_nodes(w, l, recs, ids, offs, counts) for l in range(5)]
+    torch.cuda.synchronize()
+    caps = [int(n[1][-1].item()) for n in nodes]
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    ms = []
+    for _ in range(launches):
+        e0.record()
+        for l in range(5):
+            gpu.svl_nodes(w, l, recs, ids, offs, counts, capacity=caps[l], out=nodes[l])
+        e1.record()
+        e1.synchronize()
+        ms.append(e0.elapsed_time(e1))
+    st = gpu.status()
+    print(json.dumps({"world": cfg.name, "kernel": "k_svl_nodes x 5 levels", "records": sum(caps),
+                      "per_level_records": caps, "launch_ms": [round(x, 4) for x in ms],
+                      "status": st}))
     gpu.close()
     return 0 if st == 0 else 1
 
