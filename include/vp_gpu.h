@@ -284,6 +284,13 @@ int vp_gpu_svl_nodes(vp_ctx *ctx, const vp_world_desc *world, uint32_t level,
                      const uint64_t *d_chunk_offsets, const uint32_t *d_level_counts,
                      const vp_node_out *out, void *stream);
 
+/* All five levels in one launch (out[l] for level l), high levels first, so
+ * that their few large nodes run beside the many small ones of levels 0-1. */
+int vp_gpu_svl_nodes_all(vp_ctx *ctx, const vp_world_desc *world, const void *d_splats,
+                         const uint32_t *d_chunk_ids, uint32_t n,
+                         const uint64_t *d_chunk_offsets, const uint32_t *d_level_counts,
+                         const vp_node_out out[5], void *stream);
+
 /* One draw of a frame: node `node` of level `level`, whose records go to
  * the assembled list from record `first` on (vp_host_octree_cull fills it). */
 typedef struct {

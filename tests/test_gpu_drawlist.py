@@ -34,6 +34,28 @@ def _nodes(gpu, w, recs, ids, offs, counts):
 
 @pytest.mark.parametrize("rb,mb,seed", [(2, (2, 1, 2), 1), (3, (3, 2, 1), 2), (4, (4, 1, 4), 3),
                                         (6, (2, 1, 2), 4)])
+def test_svl_nodes_all_levels_match_oracle(gpu, rb, mb, seed):
+    import torch
+
+    w = abi.WorldDesc.make(rb, mb)
+    rng = np.random.default_rng(50 + seed)
+    vox = RI.random_world(w, rng, 0.3)
+    smap = O.shadow_build(w, vox)
+    ids = rng.permutation(w.chunks).astype(np.uint32)
+    recs, offs, counts = _remesh(gpu, w, dev(vox), ids, dev(smap))
+    out = gpu.svl_nodes_all(w, recs, ids_dev(ids), offs, counts)
+    torch.cuda.synchronize()
+    assert gpu.status() == 0
+    rh, oh, ch = host(recs), host(offs, np.uint64).astype(np.int64), host(counts, np.uint32)
+    for l in range(5):
+        exp, eoffs = O.svl_nodes(w, l, rh, ids, oh, ch.astype(np.int64))
+        go = host(out[l][1], np.uint64)
+        assert np.array_equal(go, eoffs), l
+        assert np.array_equal(host(out[l][0])[: int(go[-1])], exp), l
+
+
+@pytest.mark.parametrize("rb,mb,seed", [(2, (2, 1, 2), 1), (3, (3, 2, 1), 2), (4, (4, 1, 4), 3),
+                                        (6, (2, 1, 2), 4)])
 def test_svl_nodes_match_oracle(gpu, rb, mb, seed):
     w = abi.WorldDesc.make(rb, mb)
     rng = np.random.default_rng(seed)
@@ -113,6 +135,15 @@ def test_frame_draw_list_and_splat_match_oracle(gpu):
 
     w, ids, recs, offs, counts = _frame_world(gpu)
     nodes = _nodes(gpu, w, recs, ids, offs, counts)
+    # all five levels in one launch give the same buffers
+    both = gpu.svl_nodes_all(w, recs, ids_dev(ids), offs, counts)
+    torch.cuda.synchronize()
+    assert gpu.status() == 0
+    for l in range(5):
+        assert torch.equal(both[l][1], nodes[l][1]), l
+        n = int(nodes[l][1][-1].item())
+        assert torch.equal(both[l][0][:n], nodes[l][0][:n]), l
+    del both
     rh, oh = host(recs), host(offs, np.uint64).astype(np.int64)
     ch = host(counts, np.uint32).astype(np.int64)
     enodes = [O.svl_nodes(w, l, rh, ids, oh, ch) for l in range(5)]

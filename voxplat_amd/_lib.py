@@ -331,6 +331,39 @@ class GPU:
         run("vp_gpu_svl_nodes")
         return recs, noffs
 
+    def svl_nodes_all(self, world: abi.WorldDesc, splats, ids, offsets, counts, out=None,
+                      stream=None):
+        """Node lists of all five levels in one launch.  Returns a list of five
+        (records int16[N,4], node offsets int64[nodes+1]); out: such a list
+        to reuse (its record buffers' lengths are the capacities)."""
+        torch = self.torch
+        n = int(ids.numel())
+        dev = ids.device
+        st = C.c_void_p(self._stream(stream))
+        outs = (abi.NodeOut * 5)()
+
+        def run(what):
+            _check(self.lib.vp_gpu_svl_nodes_all(
+                self.ctx, C.byref(world), C.c_void_p(_ptr(splats)), C.c_void_p(_ptr(ids)), n,
+                C.c_void_p(_ptr(offsets)), C.c_void_p(_ptr(counts)), outs, st), what)
+        if out is None:
+            noffs = [torch.empty(abi.node_count(world, l) + 1, dtype=torch.int64, device=dev)
+                     for l in range(5)]
+            for l in range(5):
+                outs[l].d_node_splats, outs[l].capacity_records = None, 0
+                outs[l].d_node_offsets = noffs[l].data_ptr()
+            run("vp_gpu_svl_nodes_all(count)")
+            caps = [int(o[-1].item()) for o in noffs]
+            out = [(torch.empty((max(c, 1), 4), dtype=torch.int16, device=dev), o)
+                   for c, o in zip(caps, noffs)]
+        for l in range(5):
+            recs, noff = out[l]
+            outs[l].d_node_splats = recs.data_ptr()
+            outs[l].capacity_records = int(recs.shape[0])
+            outs[l].d_node_offsets = noff.data_ptr()
+        run("vp_gpu_svl_nodes_all")
+        return out
+
     def draw_gather(self, world: abi.WorldDesc, nodes, node_offsets, draws, total: int,
                     out=None, stream=None):
         """Assemble a frame's list from its draws (structured array from

@@ -133,6 +133,9 @@ PROTOTYPES = {
     "vp_gpu_svl_nodes": (C.c_int, [C.c_void_p, C.POINTER(WorldDesc), C.c_uint32, C.c_void_p,
                                    C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p,
                                    C.POINTER(NodeOut), C.c_void_p]),
+    "vp_gpu_svl_nodes_all": (C.c_int, [C.c_void_p, C.POINTER(WorldDesc), C.c_void_p, C.c_void_p,
+                                       C.c_uint32, C.c_void_p, C.c_void_p, NodeOut * 5,
+                                       C.c_void_p]),
     "vp_gpu_draw_gather": (C.c_int, [C.c_void_p, C.POINTER(WorldDesc), C.c_void_p * 5,
                                      C.c_void_p * 5, C.c_void_p, C.c_uint32, C.c_void_p,
                                      C.c_void_p, C.c_uint64, C.c_void_p]),

@@ -25,19 +25,30 @@ constexpr uint64_t kNValMask = (1ull << 62) - 1;
 constexpr int kNodeBS = 512;
 constexpr int kNodeMaxSub = 4096;   // chunks of a level-4 node: 16^3
 
+// One launch builds the node buffers of up to five levels.  Tickets run
+// over the listed levels in table order (the caller lists the high levels,
+// with few large nodes, first); a node's look-back stops at its level's
+// first ticket.
+struct NodeLevel {
+    uint32_t level;
+    uint32_t nb0, nb1;           // node index bits of x, y
+    uint64_t first;              // first ticket of this level
+    uint64_t nnodes;
+    uint64_t *out;
+    uint64_t cap;
+    uint64_t *node_off;
+};
+
 struct NodeArgs {
     const uint64_t *splats;      // remesh records, 8 bytes each
     const uint32_t *cmap;        // per chunk index: list entry + 1, 0 = not listed
     const uint64_t *chunk_off;   // n+1 exclusive prefix, int16 items
     const uint32_t *lcounts;     // n*5, int16 items
-    uint32_t level;
     uint32_t mb0, mb1;           // chunk index bits of x, y
-    uint32_t nb0, nb1;           // node index bits of x, y
     uint32_t max0, max1, max2;   // chunks per axis
-    uint64_t nnodes;
-    uint64_t *out;
-    uint64_t cap;
-    uint64_t *node_off;
+    uint32_t nlev;
+    NodeLevel lev[5];
+    uint64_t ntickets;
     uint32_t *ticket;
     uint64_t *state;
     uint32_t *status;
@@ -61,13 +72,10 @@ __global__ void k_cmap_fill(uint32_t *m, const uint32_t *ids, uint32_t n, uint32
     else atomicOr(status, (uint32_t)VP_STATUS_BAD_ID);
 }
 
-__global__ void k_node_reset(uint32_t *ticket, uint64_t *state, uint64_t n, uint64_t *offsets)
+__global__ void k_node_reset(uint32_t *ticket, uint64_t *state, uint64_t n)
 {
     const uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
-    if (i == 0) {
-        ticket[0] = 0u;
-        offsets[0] = 0ull;
-    }
+    if (i == 0) ticket[0] = 0u;
     for (uint64_t j = i; j < n; j += (uint64_t)gridDim.x * blockDim.x) state[j] = 0ull;
 }
 
@@ -106,16 +114,20 @@ __global__ void __launch_bounds__(kNodeBS) k_svl_nodes(NodeArgs a)
     __shared__ uint32_t wsum[kNodeBS / 64 + 1];
     __shared__ uint32_t s_ticket;
     __shared__ uint64_t s_base;
-    const uint32_t l = a.level;
     for (;;) {
         if (threadIdx.x == 0) s_ticket = atomicAdd(a.ticket, 1u);
         __syncthreads();
-        const uint64_t k = s_ticket;
+        const uint64_t t = s_ticket;
         __syncthreads();
-        if (k >= a.nnodes) return;
-        const uint32_t nx = (uint32_t)(k & ((1ull << a.nb0) - 1ull));
-        const uint32_t ny = (uint32_t)((k >> a.nb0) & ((1ull << a.nb1) - 1ull));
-        const uint32_t nz = (uint32_t)(k >> (a.nb0 + a.nb1));
+        if (t >= a.ntickets) return;
+        uint32_t li = 0;
+        while (li + 1 < a.nlev && t >= a.lev[li + 1].first) li++;
+        const NodeLevel &L = a.lev[li];
+        const uint32_t l = L.level;
+        const uint64_t k = t - L.first;          // node index within the level
+        const uint32_t nx = (uint32_t)(k & ((1ull << L.nb0) - 1ull));
+        const uint32_t ny = (uint32_t)((k >> L.nb0) & ((1ull << L.nb1) - 1ull));
+        const uint32_t nz = (uint32_t)(k >> (L.nb0 + L.nb1));
         // the node's chunk box (vsplat.c:213-222)
         const uint32_t x0 = nx << l, y0 = ny << l, z0 = nz << l;
         const uint32_t x1 = min((nx + 1u) << l, a.max0), y1 = min((ny + 1u) << l, a.max1),
@@ -160,14 +172,15 @@ __global__ void __launch_bounds__(kNodeBS) k_svl_nodes(NodeArgs a)
         if (threadIdx.x < 64) {
             uint64_t excl = 0;
             if (k == 0) {
-                if (threadIdx.x == 0) st_relaxed(a.state, kNFlagPre | total);
+                if (threadIdx.x == 0) st_relaxed(a.state + t, kNFlagPre | total);
             } else {
-                if (threadIdx.x == 0) st_relaxed(a.state + k, kNFlagAgg | total);
-                int64_t top = (int64_t)k - 1;
+                if (threadIdx.x == 0) st_relaxed(a.state + t, kNFlagAgg | total);
+                int64_t top = (int64_t)t - 1;
+                const int64_t lstart = (int64_t)L.first;
                 const uint32_t ln = threadIdx.x & 63u;
                 for (;;) {
                     const int64_t j = top - (int64_t)ln;
-                    const uint64_t sv = (j >= 0) ? ld_relaxed(a.state + j) : kNFlagPre;
+                    const uint64_t sv = (j >= lstart) ? ld_relaxed(a.state + j) : kNFlagPre;
                     const uint64_t f = sv >> 62;
                     const uint64_t pr = __ballot(f == 2);
                     const uint64_t zero = __ballot(f == 0);
@@ -183,18 +196,18 @@ __global__ void __launch_bounds__(kNodeBS) k_svl_nodes(NodeArgs a)
                     if (pr) break;
                     top -= 64;
                 }
-                if (threadIdx.x == 0) st_relaxed(a.state + k, kNFlagPre | (excl + total));
+                if (threadIdx.x == 0) st_relaxed(a.state + t, kNFlagPre | (excl + total));
             }
             if (threadIdx.x == 0) {
                 s_base = excl;
-                a.node_off[k] = excl;
-                if (k + 1 == a.nnodes) a.node_off[a.nnodes] = excl + total;
-                if (a.out && excl + total > a.cap) atomicOr(a.status, (uint32_t)VP_STATUS_OVERFLOW);
+                L.node_off[k] = excl;
+                if (k + 1 == L.nnodes) L.node_off[L.nnodes] = excl + total;
+                if (L.out && excl + total > L.cap) atomicOr(a.status, (uint32_t)VP_STATUS_OVERFLOW);
             }
         }
         __syncthreads();
         const uint64_t base = s_base;
-        if (a.out && total && base + total <= a.cap) {
+        if (L.out && total && base + total <= L.cap) {
             // U records per thread per step, all loads issued before any
             // store: a node of a high level is one of few workgroups, so its
             // copy has to keep many loads in flight on its own
@@ -219,7 +232,7 @@ __global__ void __launch_bounds__(kNodeBS) k_svl_nodes(NodeArgs a)
 #pragma unroll
                 for (int u = 0; u < U; u++) {
                     const uint32_t j = j0 + (uint32_t)u * kNodeBS;
-                    if (j < total) a.out[base + j] = v[u];
+                    if (j < total) L.out[base + j] = v[u];
                 }
             }
         }
@@ -278,24 +291,39 @@ __global__ void __launch_bounds__(256) k_draw_gather(GatherArgs a)
 
 using namespace vp;
 
-extern "C" int vp_gpu_svl_nodes(vp_ctx *ctx, const vp_world_desc *world, uint32_t level,
-                                const void *d_splats, const uint32_t *d_chunk_ids, uint32_t n,
-                                const uint64_t *d_chunk_offsets, const uint32_t *d_level_counts,
-                                const vp_node_out *out, void *stream)
+static int svl_nodes_launch(vp_ctx *ctx, const vp_world_desc *world, uint32_t nlev,
+                            const uint32_t *levels, const void *d_splats, const uint32_t *d_chunk_ids,
+                            uint32_t n, const uint64_t *d_chunk_offsets, const uint32_t *d_level_counts,
+                            const vp_node_out *outs, void *stream)
 {
-    if (!ctx || !world || !out || !out->d_node_offsets || level > 4) return VP_ERR_ARG;
+    if (!ctx || !world || !outs) return VP_ERR_ARG;
     if (n && (!d_splats || !d_chunk_ids || !d_chunk_offsets || !d_level_counts)) return VP_ERR_ARG;
-    const bool count_only = !out->d_node_splats && out->capacity_records == 0;
-    if (!count_only && !out->d_node_splats) return VP_ERR_ARG;
     const uint32_t mbs = (uint32_t)world->max_bitw[0] + world->max_bitw[1] + world->max_bitw[2];
     if (mbs > 31u) return VP_ERR_ARG;
-    // a level-l node covers up to 2^l chunks per axis
-    for (int ax = 0; ax < 3; ax++)
-        if ((1u << (level < world->max_bitw[ax] ? level : world->max_bitw[ax])) > 16u) return VP_ERR_ARG;
+    NodeArgs a = {};
+    a.nlev = nlev;
+    uint64_t tickets = 0;
+    for (uint32_t i = 0; i < nlev; i++) {
+        const vp_node_out *o = outs + i;
+        const uint32_t l = levels[i];
+        if (l > 4 || !o->d_node_offsets) return VP_ERR_ARG;
+        const bool count_only = !o->d_node_splats && o->capacity_records == 0;
+        if (!count_only && !o->d_node_splats) return VP_ERR_ARG;
+        NodeLevel &L = a.lev[i];
+        L.level = l;
+        L.nb0 = vp_node_bits(world, l, 0);
+        L.nb1 = vp_node_bits(world, l, 1);
+        L.first = tickets;
+        L.nnodes = vp_node_count(world, l);
+        L.out = count_only ? nullptr : static_cast<uint64_t *>(o->d_node_splats);
+        L.cap = o->capacity_records;
+        L.node_off = o->d_node_offsets;
+        tickets += L.nnodes;
+    }
     hipStream_t st = as_stream(stream);
-    const uint64_t nchunks = 1ull << mbs, nnodes = vp_node_count(world, level);
+    const uint64_t nchunks = 1ull << mbs;
     std::lock_guard<std::mutex> g(ctx->mu);
-    int rc = ensure_state(ctx, nnodes);
+    int rc = ensure_state(ctx, tickets);
     if (rc) return rc;
     rc = ensure_cmap(ctx, nchunks);
     if (rc) return rc;
@@ -307,34 +335,49 @@ extern "C" int vp_gpu_svl_nodes(vp_ctx *ctx, const vp_world_desc *world, uint32_
                            n, (uint32_t)nchunks, ctx->d_status);
         VP_HIP_TRY(hipGetLastError());
     }
-    const int rgrid = (int)((nnodes + 255) / 256 < 1024 ? (nnodes + 255) / 256 : 1024);
-    hipLaunchKernelGGL(k_node_reset, dim3(rgrid), dim3(256), 0, st, ctx->d_ticket, ctx->d_state, nnodes,
-                       out->d_node_offsets);
+    const int rgrid = (int)((tickets + 255) / 256 < 1024 ? (tickets + 255) / 256 : 1024);
+    hipLaunchKernelGGL(k_node_reset, dim3(rgrid), dim3(256), 0, st, ctx->d_ticket, ctx->d_state, tickets);
     VP_HIP_TRY(hipGetLastError());
-    NodeArgs a = {};
     a.splats = static_cast<const uint64_t *>(d_splats);
     a.cmap = ctx->d_cmap;
     a.chunk_off = d_chunk_offsets;
     a.lcounts = d_level_counts;
-    a.level = level;
     a.mb0 = world->max_bitw[0];
     a.mb1 = world->max_bitw[1];
-    a.nb0 = vp_node_bits(world, level, 0);
-    a.nb1 = vp_node_bits(world, level, 1);
     a.max0 = 1u << world->max_bitw[0];
     a.max1 = 1u << world->max_bitw[1];
     a.max2 = 1u << world->max_bitw[2];
-    a.nnodes = nnodes;
-    a.out = count_only ? nullptr : static_cast<uint64_t *>(out->d_node_splats);
-    a.cap = out->capacity_records;
-    a.node_off = out->d_node_offsets;
+    a.ntickets = tickets;
     a.ticket = ctx->d_ticket;
     a.state = ctx->d_state;
     a.status = ctx->d_status;
-    const uint64_t resident = (uint64_t)ctx->num_cus * 4u;
-    const int grid = (int)(nnodes < resident ? nnodes : resident);
+    const uint64_t resident = (uint64_t)ctx->num_cus * 3u;
+    const int grid = (int)(tickets < resident ? tickets : resident);
     hipLaunchKernelGGL(k_svl_nodes, dim3(grid), dim3(kNodeBS), 0, st, a);
     return hipGetLastError() == hipSuccess ? VP_OK : VP_ERR_HIP;
+}
+
+extern "C" int vp_gpu_svl_nodes(vp_ctx *ctx, const vp_world_desc *world, uint32_t level,
+                                const void *d_splats, const uint32_t *d_chunk_ids, uint32_t n,
+                                const uint64_t *d_chunk_offsets, const uint32_t *d_level_counts,
+                                const vp_node_out *out, void *stream)
+{
+    if (level > 4) return VP_ERR_ARG;
+    return svl_nodes_launch(ctx, world, 1, &level, d_splats, d_chunk_ids, n, d_chunk_offsets,
+                            d_level_counts, out, stream);
+}
+
+extern "C" int vp_gpu_svl_nodes_all(vp_ctx *ctx, const vp_world_desc *world, const void *d_splats,
+                                    const uint32_t *d_chunk_ids, uint32_t n,
+                                    const uint64_t *d_chunk_offsets, const uint32_t *d_level_counts,
+                                    const vp_node_out out[5], void *stream)
+{
+    // levels 4..0: the few large nodes of the high levels take their tickets first
+    const uint32_t levels[5] = {4, 3, 2, 1, 0};
+    if (!out) return VP_ERR_ARG;
+    const vp_node_out ordered[5] = {out[4], out[3], out[2], out[1], out[0]};
+    return svl_nodes_launch(ctx, world, 5, levels, d_splats, d_chunk_ids, n, d_chunk_offsets,
+                            d_level_counts, ordered, stream);
 }
 
 extern "C" int vp_gpu_draw_gather(vp_ctx *ctx, const vp_world_desc *world,
