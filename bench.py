@@ -130,7 +130,8 @@ def main() -> int:
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported "
-              "(the pipeline runs on a single MI355X; see DESIGN.md for how it would shard)",
+              "(the pipeline runs on a single MI355X; a z-slab split of the chunk grid is how it "
+              "would shard)",
               file=sys.stderr)
         return 2
     if args.workload == "world16k":
