@@ -259,9 +259,9 @@ __global__ void __launch_bounds__(kMeshBS, 4) k_mesh(MeshArgs a)
     __shared__ uint64_t s_base;
 
     const uint32_t mx = 1u << a.mb0, my = 1u << a.mb1, mz = 1u << a.mb2;
-    for (uint32_t pass = 0;; pass++) {
+    for (uint32_t visit = 0;; visit++) {
         // a single chunk takes no ticket: the one workgroup does list entry 0
-        if (threadIdx.x == 0) s_ticket = a.single ? (pass == 0 ? 0u : a.n) : atomicAdd(a.ticket, 1u);
+        if (threadIdx.x == 0) s_ticket = a.single ? (visit == 0 ? 0u : a.n) : atomicAdd(a.ticket, 1u);
         __syncthreads();
         const uint32_t k = s_ticket;
         if (k >= a.n) return;
