@@ -58,6 +58,19 @@ class ResidentWorld:
             host_shadow_spheres(self.w, m, adds)
             self.smap.copy_(torch.from_numpy(m.view(np.int16)))
 
+    def open(self, ids, words, offsets, uniform) -> None:
+        """Decode chunks `ids` from RLE streams back into the world.
+
+        The reference decodes a compressed chunk when it is opened
+        (chunk_open_ro / chunk_open_rw, src/chunkset.c:133-200) after
+        chunk_compress dropped its voxels (chunkset.c:213-230,300-306).  Here
+        the streams are the ones an encode of the idle list wrote (`words`,
+        `offsets` in words, one entry per id) and `uniform` the per-entry
+        values that encode returned; they become the chunks' uniform
+        metadata again."""
+        self.gpu.rle_decode(self.w, words, offsets, ids, self.voxels)
+        self.uniform.index_copy_(0, ids.long(), uniform)
+
     def lists(self):
         """(dirty chunk ids, every other chunk id) as int32 device tensors, ascending."""
         d = self.dirty != 0
