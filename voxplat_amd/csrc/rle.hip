@@ -232,17 +232,40 @@ __device__ void rle_encode_chunk(const RleArgs &a, uint32_t k, uint32_t n, uint6
 
     // ---- 4. words ----
     // adjacent lanes take adjacent bitmap words, so a store instruction's
-    // addresses stay within a few cache lines
+    // addresses stay within a few cache lines.  The starts of a word are taken
+    // G at a time: G value loads issued together (the chunk left the L2 since
+    // step 1, so each is a memory round trip), then their G words stored.
+    constexpr int G = 8;
     uint32_t *out = a.words + base;
     for (int w = threadIdx.x; w < WORDS; w += BS) {
         uint64_t b = bm[w];
         uint32_t o = woff[w];
+        const uint32_t wn = wnext[w];
         while (b) {
-            const int bit = __builtin_ctzll(b);
-            b &= b - 1ull;
-            const uint32_t pos = (uint32_t)(w * 64 + bit);
-            const uint32_t end = b ? (uint32_t)(w * 64 + __builtin_ctzll(b)) : wnext[w];
-            out[o++] = ((uint32_t)(src ? src[pos] : 0u) << 24) | (end - pos);
+            uint32_t pos[G], val[G];
+            int cnt = 0;
+#pragma unroll
+            for (int j = 0; j < G; j++) {
+                const bool v = b != 0ull;    // true for j = 0
+                pos[j] = v ? (uint32_t)(w * 64 + __builtin_ctzll(b)) : (j ? pos[0] : 0u);
+                b = v ? (b & (b - 1ull)) : b;
+                cnt += v ? 1 : 0;
+            }
+            const uint32_t after = b ? (uint32_t)(w * 64 + __builtin_ctzll(b)) : wn;
+            if (src) {
+#pragma unroll
+                for (int j = 0; j < G; j++) val[j] = src[pos[j]];
+            } else {
+#pragma unroll
+                for (int j = 0; j < G; j++) val[j] = 0u;
+            }
+#pragma unroll
+            for (int j = 0; j < G; j++)
+                if (j < cnt) {
+                    const uint32_t end = (j + 1 < cnt) ? pos[j + 1] : after;
+                    out[o + j] = (val[j] << 24) | (end - pos[j]);
+                }
+            o += (uint32_t)cnt;
         }
     }
     if (threadIdx.x == 0) out[runs] = 0u;
