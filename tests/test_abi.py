@@ -71,3 +71,40 @@ def test_host_shadow_spheres_matches_oracle():
     for s in spheres:
         O.edit_sphere(w, vox, b, (s.cx, s.cy, s.cz), s.radius, s.voxel)
     assert a.any() and np.array_equal(a, b)
+
+
+class _FakeDev:
+    """Stands in for a device tensor: only what the size check reads."""
+
+    def __init__(self, nbytes, cuda=True, contiguous=True):
+        self.nbytes, self.is_cuda, self._c = nbytes, cuda, contiguous
+
+    def is_contiguous(self):
+        return self._c
+
+    def numel(self):
+        return self.nbytes
+
+    def element_size(self):
+        return 1
+
+
+def test_wrapper_size_checks_reject_short_or_host_buffers():
+    from voxplat_amd import _lib, abi
+
+    _lib._need(_FakeDev(64), 64, "x")
+    _lib._need(None, 64, "x")
+    with pytest.raises(ValueError, match="needs at least 65"):
+        _lib._need(_FakeDev(64), 65, "x")
+    with pytest.raises(ValueError, match="device tensor"):
+        _lib._need(_FakeDev(64, cuda=False), 8, "x")
+    with pytest.raises(ValueError, match="device tensor"):
+        _lib._need(_FakeDev(64, contiguous=False), 8, "x")
+    w = abi.WorldDesc.make(3, (2, 1, 2))
+    sx, sz = w.shadow_dims()
+    _lib._need_world(w, _FakeDev(w.chunks * 512), _FakeDev(w.chunks * 2),
+                     _FakeDev(abi.shadow_padded_len(sx, sz) * 2))
+    with pytest.raises(ValueError, match="voxels"):
+        _lib._need_world(w, _FakeDev(w.chunks * 512 - 1))
+    with pytest.raises(ValueError, match="shadow_map"):
+        _lib._need_world(w, _FakeDev(w.chunks * 512), None, _FakeDev(sx * sz * 2))

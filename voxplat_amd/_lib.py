@@ -53,6 +53,27 @@ def _ptr(t) -> int | None:
     return t.data_ptr()
 
 
+def _need(t, nbytes: int, what: str) -> None:
+    """A device tensor handed to the library holds at least nbytes, contiguous.
+    The C ABI takes bare pointers and cannot check their sizes, so the Python
+    side checks before a kernel would read or write past a buffer."""
+    if t is None:
+        return
+    if not t.is_cuda or not t.is_contiguous():
+        raise ValueError(f"{what}: needs a contiguous device tensor")
+    have = t.numel() * t.element_size()
+    if have < nbytes:
+        raise ValueError(f"{what}: {have} bytes, needs at least {nbytes}")
+
+
+def _need_world(world, voxels, uniform=None, shadow_map=None) -> None:
+    _need(voxels, world.chunks * world.R ** 3, "voxels")
+    _need(uniform, world.chunks * 2, "uniform")
+    if shadow_map is not None:
+        sx, sz = world.shadow_dims()
+        _need(shadow_map, abi.shadow_padded_len(sx, sz) * 2, "shadow_map")
+
+
 class GPU:
     """One vp_ctx on a device, with helpers that take torch tensors."""
 
@@ -101,6 +122,11 @@ class GPU:
         torch = self.torch
         n = int(ids.numel())
         dev = voxels.device
+        _need_world(world, voxels, uniform, shadow_map)
+        _need(ids, n * 4, "ids")
+        if out is not None:
+            _need(out[0], (n + 1) * 8, "offsets")
+            _need(out[1], n * 5 * 4, "level_counts")
         flags = (abi.VP_CM_LOD if lod else 0) | (abi.VP_CM_RECORD_V2 if v2 else 0)
         sh = abi.ShadowDesc()
         if shadow_map is not None:
@@ -126,7 +152,7 @@ class GPU:
             capacity = int(offsets[n].item()) // 4
         recs = records if records is not None else \
             torch.empty((max(capacity, 1), 4), dtype=torch.int16, device=dev)
-        assert recs.shape[0] >= capacity
+        _need(recs, capacity * 8, "records")
         so.d_splats, so.capacity_records = recs.data_ptr(), capacity
         _check(self.lib.vp_gpu_cull_mesh(
             self.ctx, C.byref(world), C.c_void_p(voxels.data_ptr()), C.c_void_p(_ptr(uniform)),
@@ -143,6 +169,10 @@ class GPU:
         torch = self.torch
         n = int(ids.numel())
         dev = voxels.device
+        _need_world(world, voxels, uniform, shadow_map)
+        _need(ids, n * 4, "ids")
+        if out is not None:
+            _need(out[2], (n + 1) * 8, "face offsets")
         sh = abi.ShadowDesc()
         sx, sz = world.shadow_dims()
         sh.d_map, sh.size_x, sh.size_z = shadow_map.data_ptr(), sx, sz
@@ -171,6 +201,8 @@ class GPU:
             index = torch.empty((max(capacity, 1), 6), dtype=torch.int32, device=dev)
         else:
             capacity = min(int(geom.shape[0]), int(index.shape[0])) if capacity is None else capacity
+        _need(geom, capacity * 32, "geometry")
+        _need(index, capacity * 24, "index")
         mo.d_geometry, mo.d_index, mo.capacity_faces = geom.data_ptr(), index.data_ptr(), capacity
         run("vp_gpu_mesh")
         return geom, index, offsets
@@ -204,12 +236,16 @@ class GPU:
         torch = self.torch
         n = int(ids.numel())
         dev = voxels.device
+        _need_world(world, voxels)
+        _need(ids, n * 4, "ids")
         if out is None:
             offsets = torch.empty(n + 1, dtype=torch.int64, device=dev)
             uni = torch.empty(n, dtype=torch.int16, device=dev) if uniform_out else None
             words = None
         else:
             words, offsets, uni = out
+            _need(offsets, (n + 1) * 8, "word offsets")
+            _need(uni, n * 2, "uniform out")
         st = C.c_void_p(self._stream(stream))
 
         def run(wptr, cap, what):
@@ -226,11 +262,16 @@ class GPU:
             words = torch.empty(max(capacity, 1), dtype=torch.int32, device=dev)
         elif capacity is None:
             capacity = int(words.numel())
+        _need(words, capacity * 4, "words")
         run(words.data_ptr(), capacity, "vp_gpu_rle(encode)")
         return words, offsets, uni
 
     def rle_decode(self, world: abi.WorldDesc, words, offsets, ids, voxels, stream=None):
         n = int(ids.numel())
+        _need_world(world, voxels)
+        _need(ids, n * 4, "ids")
+        _need(offsets, (n + 1) * 8, "word offsets")
+        _need(words, 4, "words")
         _check(self.lib.vp_gpu_rle(self.ctx, abi.VP_RLE_DECODE, C.byref(world),
                                    C.c_void_p(voxels.data_ptr()), C.c_void_p(ids.data_ptr()), n,
                                    C.c_void_p(words.data_ptr()), int(words.numel()),
@@ -245,8 +286,12 @@ class GPU:
         torch = self.torch
         n = int(splats.shape[0])
         nb = (n + abi.VP_SPLAT_BLOCK - 1) // abi.VP_SPLAT_BLOCK
+        _need(splats, n * 8, "splats")
+        _need(lod, n, "lod")
         if out is not None:
-            assert out.shape[0] >= nb and out.dtype == torch.float32
+            if out.dtype != torch.float32:
+                raise ValueError("bounds: needs float32")
+            _need(out, nb * 32, "bounds")
             b = out[:max(nb, 1)]
         else:
             b = torch.empty((max(nb, 1), 8), dtype=torch.float32, device=splats.device)
@@ -270,6 +315,12 @@ class GPU:
             wt = torch.empty((H, W), dtype=torch.float32, device=dev) if weight else None
         else:
             depth, ids, col, wt = out
+        _need(splats, n * 8, "splats")
+        _need(lod, n, "lod")
+        _need(voxel_rgb, n * 12, "voxel_rgb")
+        _need(bounds, ((n + abi.VP_SPLAT_BLOCK - 1) // abi.VP_SPLAT_BLOCK) * 32, "bounds")
+        for t, per, what in ((depth, 4, "depth"), (ids, 4, "ids"), (col, 12, "rgb"), (wt, 4, "weight")):
+            _need(t, W * H * per, what)
         fr = abi.Frame()
         fr.d_depth, fr.d_id, fr.d_rgb, fr.d_weight = (depth.data_ptr(), ids.data_ptr(),
                                                       _ptr(col), _ptr(wt))
@@ -288,6 +339,10 @@ class GPU:
         return depth, ids, col, wt
 
     def splat_backward(self, n: int, cam: abi.Camera, ids, weight, dL_drgb, grad, stream=None):
+        np_ = cam.width * cam.height
+        for t, nb, what in ((ids, np_ * 4, "ids"), (weight, np_ * 4, "weight"),
+                            (dL_drgb, np_ * 12, "dL_drgb"), (grad, n * 12, "grad")):
+            _need(t, nb, what)
         _check(self.lib.vp_gpu_splat_backward(self.ctx, n, C.byref(cam),
                                               C.c_void_p(ids.data_ptr()),
                                               C.c_void_p(weight.data_ptr()),
@@ -307,10 +362,14 @@ class GPU:
         n = int(ids.numel())
         dev = ids.device
         nn = abi.node_count(world, level)
+        _need(ids, n * 4, "ids")
+        _need(offsets, (n + 1) * 8, "chunk offsets")
+        _need(counts, n * 5 * 4, "level counts")
         if out is None:
             recs, noffs = None, torch.empty(nn + 1, dtype=torch.int64, device=dev)
         else:
             recs, noffs = out
+            _need(noffs, (nn + 1) * 8, "node offsets")
         st = C.c_void_p(self._stream(stream))
         no = abi.NodeOut()
         no.d_node_offsets = noffs.data_ptr()
@@ -327,6 +386,7 @@ class GPU:
             recs = torch.empty((max(capacity, 1), 4), dtype=torch.int16, device=dev)
         elif capacity is None:
             capacity = int(recs.shape[0])
+        _need(recs, capacity * 8, "node records")
         no.d_node_splats, no.capacity_records = recs.data_ptr(), capacity
         run("vp_gpu_svl_nodes")
         return recs, noffs
@@ -341,6 +401,9 @@ class GPU:
         dev = ids.device
         st = C.c_void_p(self._stream(stream))
         outs = (abi.NodeOut * 5)()
+        _need(ids, n * 4, "ids")
+        _need(offsets, (n + 1) * 8, "chunk offsets")
+        _need(counts, n * 5 * 4, "level counts")
 
         def run(what):
             _check(self.lib.vp_gpu_svl_nodes_all(
@@ -358,6 +421,8 @@ class GPU:
                    for c, o in zip(caps, noffs)]
         for l in range(5):
             recs, noff = out[l]
+            _need(recs, int(recs.shape[0]) * 8, "node records")
+            _need(noff, (abi.node_count(world, l) + 1) * 8, "node offsets")
             outs[l].d_node_splats = recs.data_ptr()
             outs[l].capacity_records = int(recs.shape[0])
             outs[l].d_node_offsets = noff.data_ptr()
@@ -376,6 +441,9 @@ class GPU:
             lod = torch.empty(max(total, 1), dtype=torch.uint8, device=dev)
         else:
             recs, lod = out
+        _need(lod, int(recs.shape[0]), "lod")
+        for l in range(5):
+            _need(node_offsets[l], (abi.node_count(world, l) + 1) * 8, "node offsets")
         d = torch.from_numpy(np.ascontiguousarray(draws).view(np.uint8).copy()).to(dev)
         nptr = (C.c_void_p * 5)(*[t.data_ptr() for t in nodes])
         optr = (C.c_void_p * 5)(*[t.data_ptr() for t in node_offsets])
