@@ -331,6 +331,13 @@ def run_world16k(frames: int = 5, warmup: int = 1, edits: int = 10000, log=None,
         rl += ev[1].elapsed_time(ev[2])
     wall = (time.perf_counter() - tw) * 1e3 / frames
     rm, rl = rm / frames, rl / frames
+    # reopen: every idle chunk decoded back in place from the stream just
+    # written (the bytes it writes are the bytes already there)
+    ev[0].record(st)
+    world.open(idle, fr.words, fr.word_offsets, fr.idle_uniform)
+    ev[1].record(st)
+    ev[1].synchronize()
+    ro = ev[0].elapsed_time(ev[1])
     status = gpu.status()
     nd, ni = int(dirty.numel()), int(idle.numel())
     dirty_uni = world.uniform[dirty.long()].cpu().numpy()
@@ -351,6 +358,8 @@ def run_world16k(frames: int = 5, warmup: int = 1, edits: int = 10000, log=None,
         "rle_gvoxels_s": round(ni * world.R3 / (rl * 1e-3) / 1e9, 2) if rl else None,
         "rle_gb_s": round(lb / (rl * 1e-3) / 1e9, 1) if rl else None,
         "frame_gb_s": round((rb + lb) / ((rm + rl) * 1e-3) / 1e9, 1),
+        "reopen_ms": round(ro, 3),
+        "reopen_gb_s": round(lb / (ro * 1e-3) / 1e9, 1) if ro else None,
         "status": status,
         "full_remesh": full,
         "on_generated": extra,
