@@ -14,9 +14,11 @@
 //  3. the chunk's word count is published and its output offset taken by a
 //     decoupled look-back (same scheme as the remesh kernel).
 //  4. each thread writes the words of its bitmap words.
-// Decode, one workgroup per chunk: run lengths are scanned into start
-// positions, set as bits of an LDS bitmap; an output byte's run is the rank of
-// the last start at or before it, so every 16-byte store is independent.
+// Decode, one workgroup per chunk: a stream of one full-chunk run is a fill;
+// otherwise the words are read eight per thread at a time, their values kept
+// in LDS and their lengths scanned into start positions, set as bits of an
+// LDS bitmap; an output byte's run is the rank of the last start at or before
+// it, so every 16-byte store is independent.
 #include "vp_internal.h"
 
 namespace vp {
@@ -299,8 +301,12 @@ template <int B, int BS>
 __global__ void __launch_bounds__(BS) k_rle_decode(RleArgs a)
 {
     constexpr int R = 1 << B, VOX = R * R * R, WORDS = VOX / 64, WPT = (WORDS + BS - 1) / BS;
+    constexpr int K = 8;             // stream words per thread per batch, loads in flight
+    constexpr int VCAP = 16384;      // run values kept in LDS for streams of up to VCAP runs
+    constexpr uint32_t LMAX = (uint32_t)VOX + 1u;   // longer runs are malformed; clamped
     __shared__ uint64_t bm[WORDS];
     __shared__ uint32_t rank[WORDS];
+    __shared__ uint8_t val8[VCAP];
     __shared__ uint32_t wsum[BS / 64 + 1];
     __shared__ uint32_t s_bad, s_nz;
     for (uint32_t k = blockIdx.x; k < a.n; k += gridDim.x) {
@@ -313,31 +319,67 @@ __global__ void __launch_bounds__(BS) k_rle_decode(RleArgs a)
         uint8_t *dst = a.voxels + ((size_t)chunk << (3 * B));
         const uint32_t *wd = a.words + o0;
         const uint64_t avail = o1 - o0;
+        // one run over the whole chunk, then the terminator: a fill
+        if (avail >= 2) {
+            const uint32_t w0 = wd[0], w1 = wd[1];   // the same address in every lane
+            if ((w0 & 0xFFFFFFu) == (uint32_t)VOX && w1 == 0u) {
+                const uint32_t b = w0 >> 24;
+                const uint32_t v4 = b * 0x01010101u;
+                const uint4 q = make_uint4(v4, v4, v4, v4);
+                for (int s = threadIdx.x; s < VOX / 16; s += BS)
+                    *reinterpret_cast<uint4 *>(dst + (size_t)s * 16) = q;
+                continue;   // uniform across the workgroup
+            }
+        }
         for (int w = threadIdx.x; w < WORDS; w += BS) bm[w] = 0ull;
-        if (threadIdx.x == 0) s_bad = 0u;
-        __syncthreads();
         // the runs end at the first 0 word after word 0 (rle.c:100-108: the
         // first word is always expanded)
-        if (threadIdx.x == 0) s_nz = (uint32_t)(avail < 0xFFFFFFFFull ? avail : 0xFFFFFFFFull);
+        if (threadIdx.x == 0) {
+            s_bad = 0u;
+            s_nz = (uint32_t)(avail < 0xFFFFFFFFull ? avail : 0xFFFFFFFFull);
+        }
         __syncthreads();
-        for (uint64_t i = 1 + threadIdx.x; i < avail; i += BS)
-            if (wd[i] == 0u) atomicMin(&s_nz, (uint32_t)i);
-        __syncthreads();
-        const uint32_t nw = s_nz;
-        // run starts: scan of run lengths, one bit per start
-        uint32_t pos = 0;
-        for (uint32_t t0 = 0; t0 < nw; t0 += BS) {
-            const uint32_t i = t0 + threadIdx.x;
-            const uint32_t len = (i < nw) ? (wd[i] & 0xFFFFFFu) : 0u;
-            if (i < nw && len == 0u) s_bad = 1u;     // zero-length runs are malformed
+        uint64_t nz = s_nz;
+        // run starts: batches of K words per thread; the terminator search,
+        // the run values (into LDS) and a scan of the lengths into start bits
+        uint64_t pos = 0;
+        for (uint64_t b0 = 0; b0 < nz; b0 += (uint64_t)BS * K) {
+            const uint64_t i0 = b0 + (uint64_t)threadIdx.x * K;
+            uint32_t wv[K];
+#pragma unroll
+            for (int j = 0; j < K; j++) wv[j] = (i0 + j < avail) ? wd[i0 + j] : 1u;
+#pragma unroll
+            for (int j = 0; j < K; j++)
+                if (i0 + j >= 1 && i0 + j < avail && i0 + j <= 0xFFFFFFFFull && wv[j] == 0u)
+                    atomicMin(&s_nz, (uint32_t)(i0 + j));
+            __syncthreads();
+            nz = s_nz;
+            uint32_t len[K], lsum = 0;
+            bool bad = false;
+#pragma unroll
+            for (int j = 0; j < K; j++) {
+                const uint64_t i = i0 + j;
+                uint32_t l = (i < nz) ? (wv[j] & 0xFFFFFFu) : 0u;
+                bad |= (i < nz) && (l == 0u || l > LMAX);   // zero-length or oversized runs
+                l = l < LMAX ? l : LMAX;
+                len[j] = l;
+                lsum += l;
+                if (i < nz && i < (uint64_t)VCAP) val8[i] = (uint8_t)(wv[j] >> 24);
+            }
+            if (bad) s_bad = 1u;
             uint32_t tot = 0;
-            const uint32_t st = pos + r_scan_excl<BS>(len, wsum, tot);
-            if (len && st < (uint32_t)VOX)
-                atomicOr(reinterpret_cast<unsigned long long *>(&bm[st >> 6]), 1ull << (st & 63u));
+            uint64_t st = pos + r_scan_excl<BS>(lsum, wsum, tot);
+#pragma unroll
+            for (int j = 0; j < K; j++) {
+                if (len[j] && st < (uint64_t)VOX)
+                    atomicOr(reinterpret_cast<unsigned long long *>(&bm[st >> 6]), 1ull << (st & 63u));
+                st += len[j];
+            }
             pos += tot;
         }
-        if (threadIdx.x == 0 && pos != (uint32_t)VOX) s_bad = 1u;
+        if (threadIdx.x == 0 && pos != (uint64_t)VOX) s_bad = 1u;
         __syncthreads();
+        const uint64_t nw = nz;
         // rank of each bitmap word = starts before it
         uint32_t mine = 0;
         const int w0 = (int)threadIdx.x * WPT;
@@ -351,20 +393,23 @@ __global__ void __launch_bounds__(BS) k_rle_decode(RleArgs a)
                 run += (uint32_t)__popcll(bm[w0 + j]);
             }
         __syncthreads();
-        // bytes: run index of byte p = starts at or before p, minus one
+        // bytes: run index of byte p = starts at or before p, minus one; each
+        // start belongs to a run index below nw
+        const bool in_lds = nw <= (uint64_t)VCAP;
         for (int s = threadIdx.x; s < VOX / 16; s += BS) {
             const int w = s >> 2;
             const uint64_t b = bm[w];
             const int sh = (s & 3) * 16;
             const uint64_t below = sh ? (b & ((1ull << sh) - 1ull)) : 0ull;
-            int ri = (int)(rank[w] + (uint32_t)__popcll(below)) - 1;
+            const int r0 = (int)(rank[w] + (uint32_t)__popcll(below)) - 1;
             const uint32_t bits16 = (uint32_t)(b >> sh) & 0xFFFFu;
             uint32_t out[4] = {0, 0, 0, 0};
-            const uint32_t p0 = (uint32_t)s * 16;
+            const uint64_t p0 = (uint64_t)s * 16;
+#pragma unroll
             for (int j = 0; j < 16; j++) {
-                if ((bits16 >> j) & 1u) ri++;
+                const int ri = r0 + __popc(bits16 & ((2u << j) - 1u));
                 uint32_t v = 0;
-                if (ri >= 0 && p0 + j < pos) v = wd[ri] >> 24;
+                if (ri >= 0 && p0 + j < pos) v = in_lds ? (uint32_t)val8[ri] : (wd[ri] >> 24);
                 out[j >> 2] |= v << ((j & 3) * 8);
             }
             *reinterpret_cast<uint4 *>(dst + p0) = make_uint4(out[0], out[1], out[2], out[3]);
