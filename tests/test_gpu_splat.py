@@ -116,11 +116,14 @@ def test_block_bounds(gpu, terrain):
     cen = recs[:, :3].astype(np.float64) + half[:, None]
     nb = (n + 255) // 256
     assert b.shape == (nb, 8)
-    for k in (0, 1, nb // 2, nb - 1):
-        s = slice(256 * k, min(n, 256 * (k + 1)))
-        assert np.array_equal(b[k, :3], cen[s].min(0).astype(np.float32))
-        assert b[k, 3] == np.float32(half[s].max())
-        assert np.array_equal(b[k, 4:7], cen[s].max(0).astype(np.float32))
+    pad = nb * 256 - n
+    lo = np.concatenate([cen, np.full((pad, 3), np.inf)]).reshape(nb, 256, 3).min(1)
+    hi = np.concatenate([cen, np.full((pad, 3), -np.inf)]).reshape(nb, 256, 3).max(1)
+    hh = np.concatenate([half, np.full(pad, -np.inf)]).reshape(nb, 256).max(1)
+    assert np.array_equal(b[:, :3], lo.astype(np.float32))       # every block, the last one partial
+    assert np.array_equal(b[:, 3], hh.astype(np.float32))
+    assert np.array_equal(b[:, 4:7], hi.astype(np.float32))
+    assert (b[:, 7] == 0).all()
 
 
 def test_block_path_matches_oracle(gpu, terrain):

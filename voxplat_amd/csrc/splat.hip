@@ -253,35 +253,48 @@ constexpr int kBins = 256;
 constexpr int kTotal = 2 * kBins;
 constexpr int kCounters = kTotal + 1;
 
+// One wave per block of kBlock splats (lane l takes splats l, l + 64, ...),
+// blocks strided over the grid's waves; a wave-wide min reduction, no LDS.
 __global__ void __launch_bounds__(256) k_bounds(const int16_t *splats, const uint8_t *lod,
-                                                uint32_t lod_log2, uint64_t n, float *bounds)
+                                                uint32_t lod_log2, uint64_t n, uint64_t nb,
+                                                float *bounds)
 {
-    __shared__ float red[8][7];
-    const uint64_t b = blockIdx.x, i = b * kBlock + threadIdx.x;
-    float v[7];   // min x y z, max half, -max x y z (all reduced with min)
-    if (i < n) {
-        const float half = 0.5f * (float)(1u << (lod ? lod[i] : lod_log2));
-        const int16_t *r = splats + 4 * i;
-        const float cx = (float)r[0] + half, cy = (float)r[1] + half, cz = (float)r[2] + half;
-        v[0] = cx; v[1] = cy; v[2] = cz; v[3] = -half; v[4] = -cx; v[5] = -cy; v[6] = -cz;
-    } else {
+    constexpr int PER = kBlock / 64;
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint64_t nwaves = (uint64_t)gridDim.x * (blockDim.x >> 6);
+    for (uint64_t b = (uint64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); b < nb;
+         b += nwaves) {
+        float v[7];   // min x y z, max half, -max x y z (all reduced with min)
         for (int k = 0; k < 7; k++) v[k] = INFINITY;
-    }
-    for (int k = 0; k < 7; k++)
-        for (int d = 32; d >= 1; d >>= 1) v[k] = fminf(v[k], __shfl_xor(v[k], d, 64));
-    const int w = (int)(threadIdx.x >> 6);
-    if ((threadIdx.x & 63u) == 0)
-        for (int k = 0; k < 7; k++) red[w][k] = v[k];
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        float m[7];
-        for (int k = 0; k < 7; k++) {
-            m[k] = red[0][k];
-            for (int j = 1; j < (int)(blockDim.x >> 6); j++) m[k] = fminf(m[k], red[j][k]);
+        uint2 r[PER];
+        uint32_t lv[PER];
+#pragma unroll
+        for (int q = 0; q < PER; q++) {
+            const uint64_t i = b * kBlock + (uint64_t)q * 64 + lane;
+            const bool in = i < n;
+            r[q] = in ? *reinterpret_cast<const uint2 *>(splats + 4 * i) : make_uint2(0u, 0u);
+            lv[q] = (in && lod) ? (uint32_t)lod[i] : lod_log2;
         }
-        float *o = bounds + 8 * b;
-        o[0] = m[0]; o[1] = m[1]; o[2] = m[2]; o[3] = -m[3];
-        o[4] = -m[4]; o[5] = -m[5]; o[6] = -m[6]; o[7] = 0.0f;
+#pragma unroll
+        for (int q = 0; q < PER; q++) {
+            const uint64_t i = b * kBlock + (uint64_t)q * 64 + lane;
+            if (i < n) {
+                const float half = 0.5f * (float)(1u << lv[q]);
+                const float cx = (float)(int16_t)(r[q].x & 0xFFFFu) + half;
+                const float cy = (float)(int16_t)(r[q].x >> 16) + half;
+                const float cz = (float)(int16_t)(r[q].y & 0xFFFFu) + half;
+                v[0] = fminf(v[0], cx); v[1] = fminf(v[1], cy); v[2] = fminf(v[2], cz);
+                v[3] = fminf(v[3], -half);
+                v[4] = fminf(v[4], -cx); v[5] = fminf(v[5], -cy); v[6] = fminf(v[6], -cz);
+            }
+        }
+        for (int k = 0; k < 7; k++)
+            for (int d = 32; d >= 1; d >>= 1) v[k] = fminf(v[k], __shfl_xor(v[k], d, 64));
+        if (lane == 0) {
+            float4 *o = reinterpret_cast<float4 *>(bounds + 8 * b);
+            o[0] = make_float4(v[0], v[1], v[2], -v[3]);
+            o[1] = make_float4(-v[4], -v[5], -v[6], 0.0f);
+        }
     }
 }
 
@@ -594,8 +607,10 @@ extern "C" int vp_gpu_splat_bounds(vp_ctx *ctx, const int16_t *d_splats, const u
     if (n > 0xFFFFFFFFull) return VP_ERR_UNSUPPORTED;
     if (n == 0) return VP_OK;
     const uint64_t nb = (n + kBlock - 1) / kBlock;
-    hipLaunchKernelGGL(k_bounds, dim3((uint32_t)nb), dim3(kBlock), 0, as_stream(stream), d_splats,
-                       d_lod, lod_log2, n, d_bounds);
+    // four blocks per workgroup (a wave each), at most 32 workgroups per CU
+    const uint64_t wgs = (nb + 3) / 4, cap = (uint64_t)ctx->num_cus * 32u;
+    hipLaunchKernelGGL(k_bounds, dim3((uint32_t)(wgs < cap ? wgs : cap)), dim3(256), 0,
+                       as_stream(stream), d_splats, d_lod, lod_log2, n, nb, d_bounds);
     VP_HIP_TRY(hipGetLastError());
     return VP_OK;
 }
