@@ -13,8 +13,9 @@
 //   then every thread copies records, finding its chunk by a binary search
 //   over the chunk prefixes in LDS, so a node of thousands of small chunks
 //   is still one coalesced copy.
-// k_draw_gather: one workgroup per draw, copying the node's records (and
-//   writing their level) to the draw's place in the frame's list.
+// k_draw_gather: eight workgroups per draw, each copying a slice of the
+//   node's records (and writing their level) to the draw's place in the
+//   frame's list.
 #include "vp_internal.h"
 
 namespace vp {
@@ -24,6 +25,7 @@ constexpr uint64_t kNFlagPre = 2ull << 62;
 constexpr uint64_t kNValMask = (1ull << 62) - 1;
 constexpr int kNodeBS = 512;
 constexpr int kNodeMaxSub = 4096;   // chunks of a level-4 node: 16^3
+constexpr int kGatherSlices = 8;    // workgroups per draw in k_draw_gather
 
 // One launch builds the node buffers of up to five levels.  Tickets run
 // over the listed levels in table order (the caller lists the high levels,
@@ -252,8 +254,12 @@ struct GatherArgs {
     uint32_t *status;
 };
 
+// Draw d's records are cut into gridDim.y equal slices; workgroup (x, y)
+// copies slice y of draws x, x + gridDim.x, ...: a near chunk's tens of
+// thousands of records are not left to a single workgroup.
 __global__ void __launch_bounds__(256) k_draw_gather(GatherArgs a)
 {
+    const uint64_t ns = gridDim.y, sl = blockIdx.y;
     for (uint32_t d = blockIdx.x; d < a.n; d += gridDim.x) {
         const uint32_t l = a.draws[d].level, node = a.draws[d].node;
         const uint64_t first = a.draws[d].first;
@@ -263,22 +269,23 @@ __global__ void __launch_bounds__(256) k_draw_gather(GatherArgs a)
         }
         const uint64_t s0 = a.noff[l][node], cnt = a.noff[l][node + 1] - s0;
         if (first + cnt > a.cap) {
-            if (threadIdx.x == 0) atomicOr(a.status, (uint32_t)VP_STATUS_OVERFLOW);
+            if (threadIdx.x == 0 && sl == 0) atomicOr(a.status, (uint32_t)VP_STATUS_OVERFLOW);
             continue;
         }
         const uint64_t *src = a.nodes[l] + s0;
+        const uint64_t jb = cnt * sl / ns, je = cnt * (sl + 1) / ns;   // this workgroup's slice
         constexpr int U = 4;                    // loads in flight per thread
-        for (uint64_t j0 = threadIdx.x; j0 < cnt; j0 += U * blockDim.x) {
+        for (uint64_t j0 = jb + threadIdx.x; j0 < je; j0 += U * blockDim.x) {
             uint64_t v[U];
 #pragma unroll
             for (int u = 0; u < U; u++) {
                 const uint64_t j = j0 + (uint64_t)u * blockDim.x;
-                v[u] = j < cnt ? src[j] : 0ull;
+                v[u] = j < je ? src[j] : 0ull;
             }
 #pragma unroll
             for (int u = 0; u < U; u++) {
                 const uint64_t j = j0 + (uint64_t)u * blockDim.x;
-                if (j < cnt) {
+                if (j < je) {
                     a.out[first + j] = v[u];
                     a.lod[first + j] = (uint8_t)l;
                 }
@@ -402,6 +409,6 @@ extern "C" int vp_gpu_draw_gather(vp_ctx *ctx, const vp_world_desc *world,
     a.cap = capacity_records;
     a.status = ctx->d_status;
     const int grid = (int)(n < 65535u ? n : 65535u);
-    hipLaunchKernelGGL(k_draw_gather, dim3(grid), dim3(256), 0, as_stream(stream), a);
+    hipLaunchKernelGGL(k_draw_gather, dim3(grid, kGatherSlices), dim3(256), 0, as_stream(stream), a);
     return hipGetLastError() == hipSuccess ? VP_OK : VP_ERR_HIP;
 }
