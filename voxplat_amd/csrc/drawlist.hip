@@ -13,7 +13,7 @@
 //   then every thread copies records, finding its chunk by a binary search
 //   over the chunk prefixes in LDS, so a node of thousands of small chunks
 //   is still one coalesced copy.
-// k_draw_gather: eight workgroups per draw, each copying a slice of the
+// k_draw_gather: sixteen workgroups per draw, each copying a slice of the
 //   node's records (and writing their level) to the draw's place in the
 //   frame's list.
 #include "vp_internal.h"
@@ -25,7 +25,7 @@ constexpr uint64_t kNFlagPre = 2ull << 62;
 constexpr uint64_t kNValMask = (1ull << 62) - 1;
 constexpr int kNodeBS = 512;
 constexpr int kNodeMaxSub = 4096;   // chunks of a level-4 node: 16^3
-constexpr int kGatherSlices = 8;    // workgroups per draw in k_draw_gather
+constexpr int kGatherSlices = 16;   // workgroups per draw in k_draw_gather
 
 // One launch builds the node buffers of up to five levels.  Tickets run
 // over the listed levels in table order (the caller lists the high levels,
