@@ -165,7 +165,7 @@ __device__ __forceinline__ void fragment(const SplatCam &c, const float4 *rays, 
     if (key < cur) atomicMin(reinterpret_cast<unsigned long long *>(frame + p), key);
 }
 
-constexpr int kSmallArea = 16;
+constexpr int kSmallArea = 32;   // 16 and 48 measured 5% slower at 1080p
 
 // Splat i (i >= n: none) of one lane; every lane of the wave calls it.
 // Sprites larger than kSmallArea pixels are shared by the whole wave, one at
