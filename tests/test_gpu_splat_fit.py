@@ -134,3 +134,27 @@ def test_adam_fit_recovers_visible_colours(gpu, scene):
     assert err < 0.05 * err0, (err0, err)
     # colours no view sees receive no gradient and stay where they started
     assert torch.equal(rgb[~seen], init[~seen].clamp(0, 1))
+
+
+def test_render_with_block_bounds_matches_plain(gpu, scene):
+    """The fit renders through the block-culled resolve (bounds computed once
+    per list): images equal the plain resolve's, gradients agree (float
+    atomics accumulate in either order)."""
+    import torch
+    from voxplat_amd import splat_fit as F
+
+    w, splats, cams = scene
+    n = splats.shape[0]
+    g = torch.Generator(device="cuda").manual_seed(5)
+    rgb = torch.rand((n, 3), generator=g, device="cuda")
+    bounds = gpu.splat_bounds(splats)
+    for cam in cams[:3]:
+        a = rgb.clone().requires_grad_(True)
+        b = rgb.clone().requires_grad_(True)
+        ia = F.render(gpu, a, splats, cam)
+        ib = F.render(gpu, b, splats, cam, bounds=bounds)
+        assert torch.equal(ia, ib)
+        G = torch.rand_like(ia)
+        ia.backward(G)
+        ib.backward(G)
+        assert torch.allclose(a.grad, b.grad, rtol=1e-5, atol=1e-6)

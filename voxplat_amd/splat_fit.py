@@ -33,12 +33,13 @@ class SplatColour(torch.autograd.Function):
     """image[H, W, 3] = splat(rgb[n, 3]); gradient w.r.t. rgb only."""
 
     @staticmethod
-    def forward(ctx, rgb, gpu: GPU, splats, cam: abi.Camera, lod=None, lod_log2: int = 0):
+    def forward(ctx, rgb, gpu: GPU, splats, cam: abi.Camera, lod=None, lod_log2: int = 0,
+                bounds=None):
         if rgb.dim() != 2 or rgb.shape[1] != 3 or rgb.shape[0] != splats.shape[0]:
             raise ValueError(f"rgb must be [{splats.shape[0]}, 3], got {tuple(rgb.shape)}")
         rgb_c = rgb.detach().to(torch.float32).contiguous()
         _, ids, img, wt = gpu.splat(splats, cam, lod=lod, lod_log2=lod_log2, voxel_rgb=rgb_c,
-                                    rgb=True, weight=True)
+                                    rgb=True, weight=True, bounds=bounds)
         ctx.save_for_backward(ids, wt)
         ctx.gpu, ctx.cam, ctx.n, ctx.dtype = gpu, cam, rgb.shape[0], rgb.dtype
         return img.to(rgb.dtype)
@@ -48,11 +49,13 @@ class SplatColour(torch.autograd.Function):
         ids, wt = ctx.saved_tensors
         grad = torch.zeros((ctx.n, 3), dtype=torch.float32, device=g_img.device)
         ctx.gpu.splat_backward(ctx.n, ctx.cam, ids, wt, g_img.to(torch.float32).contiguous(), grad)
-        return grad.to(ctx.dtype), None, None, None, None, None
+        return grad.to(ctx.dtype), None, None, None, None, None, None
 
 
-def render(gpu: GPU, rgb, splats, cam, lod=None, lod_log2: int = 0):
-    return SplatColour.apply(rgb, gpu, splats, cam, lod, lod_log2)
+def render(gpu: GPU, rgb, splats, cam, lod=None, lod_log2: int = 0, bounds=None):
+    """bounds: the list's block bounds (GPU.splat_bounds), computed once while
+    the splats stay put; the resolve then culls and orders blocks per view."""
+    return SplatColour.apply(rgb, gpu, splats, cam, lod, lod_log2, bounds)
 
 
 def build_scene(gpu: GPU, cfg: W.WorldConfig = W.CFG4):
@@ -87,7 +90,7 @@ def views(w: abi.WorldDesc, vox, n: int, size: int, seed: int):
 
 
 def fit_colours(gpu: GPU, splats, cams, targets, init, steps: int = 200, lr: float = 0.05,
-                log_every: int = 0):
+                log_every: int = 0, bounds=None):
     """Adam on per-splat RGB from `init`, loss = mean over views of the mean
     squared pixel error; colours are clamped to [0, 1] after each step.
     Returns (rgb, per-step loss list)."""
@@ -97,7 +100,7 @@ def fit_colours(gpu: GPU, splats, cams, targets, init, steps: int = 200, lr: flo
     for it in range(steps):
         opt.zero_grad(set_to_none=True)
         for cam, tgt in zip(cams, targets):
-            img = render(gpu, rgb, splats, cam)
+            img = render(gpu, rgb, splats, cam, bounds=bounds)
             l = ((img - tgt) ** 2).mean() / len(cams)
             l.backward()
             hist[it] += l.detach()
@@ -109,11 +112,11 @@ def fit_colours(gpu: GPU, splats, cams, targets, init, steps: int = 200, lr: flo
     return rgb.detach(), hist.tolist()
 
 
-def visible(gpu: GPU, splats, cams):
+def visible(gpu: GPU, splats, cams, bounds=None):
     """Mask of the splats that are the nearest hit of at least one pixel."""
     seen = torch.zeros(splats.shape[0], dtype=torch.bool, device=splats.device)
     for c in cams:
-        _, ids, _, _ = gpu.splat(splats, c)
+        _, ids, _, _ = gpu.splat(splats, c, bounds=bounds)
         hit = ids[ids >= 0]
         seen[hit.long()] = True
     return seen
@@ -137,14 +140,15 @@ def main(argv=None) -> int:
     g = torch.Generator(device="cuda").manual_seed(cfg.seed)
     gt = torch.rand((n, 3), generator=g, device="cuda")
     init = torch.rand((n, 3), generator=g, device="cuda")
+    bounds = gpu.splat_bounds(splats)
     with torch.no_grad():
-        targets = [render(gpu, gt, splats, c) for c in cams]
-    seen = visible(gpu, splats, cams)
+        targets = [render(gpu, gt, splats, c, bounds=bounds) for c in cams]
+    seen = visible(gpu, splats, cams, bounds)
     torch.cuda.synchronize()
     t_setup = time.perf_counter() - t0
     t0 = time.perf_counter()
     rgb, hist = fit_colours(gpu, splats, cams, targets, init, steps=a.steps, lr=a.lr,
-                            log_every=20)
+                            log_every=20, bounds=bounds)
     torch.cuda.synchronize()
     t_fit = time.perf_counter() - t0
     print(json.dumps({
