@@ -146,3 +146,31 @@ def test_block_path_everything_culled_or_empty(gpu):
     cam = W.make_camera((300.0, 10.0, 300.0), 180.0, 0.0, 64, 48)   # looking away from the row
     ids, depth = _compare(gpu, splats, cam, blocks=True)
     assert (ids == 0xFFFFFFFF).all() and (depth == 1.0).all()
+
+
+@pytest.mark.parametrize("blocks", [False, True])
+def test_sprite_footprint_closed_form(gpu, blocks):
+    """The sprite cases the oracle is pinned with (window = hit set, the
+    half-open window edge, the far-plane entry test), rendered by the GPU
+    and compared with the closed-form expectation as well as the oracle."""
+    from tests import test_oracle_splat as T
+
+    for case in range(len(T._SPRITE_CASES)):
+        pos, loc, yaw, pitch, fov, Wd, Ht = T._case(case)
+        cam = W.make_camera(loc, yaw, pitch, Wd, Ht, fov_deg=fov)
+        lo_x, hi_x, lo_y, hi_y, _ = T._sprite_window(pos, loc, yaw, pitch, fov, Wd, Ht)
+        want = T._covered(lo_y, hi_y, Ht)[:, None] & T._covered(lo_x, hi_x, Wd)[None, :]
+        ids, _ = _compare(gpu, np.array([[*pos, 45]], np.int16), cam, blocks=blocks)
+        assert np.array_equal(ids != 0xFFFFFFFF, want), case
+    cam = W.make_camera((7.5, 9.5, 111.5), 0.0, 0.0, 64, 64, fov_deg=10.0)
+    ids, _ = _compare(gpu, np.array([[7, 9, 11, 7]], np.int16), cam, blocks=blocks)
+    assert np.argwhere(ids != 0xFFFFFFFF).tolist() == [[31, 31]]
+    # 256-voxel splats 25 degrees off the axis: box entry beyond far, then inside
+    loc = (2000.5, 300.5, 8000.5)
+    cam = W.make_camera(loc, 0.0, 0.0, 64, 64)
+    f = np.tan(np.radians(25.0))
+    for zc, any_hit in ((4000.0, False), (3600.0, True)):
+        centre = (loc[0] + f * zc, loc[1], loc[2] - zc)
+        pos = [int(round(c - 128)) for c in centre]
+        ids, _ = _compare(gpu, np.array([[*pos, 9]], np.int16), cam, lod_log2=8, blocks=blocks)
+        assert (ids != 0xFFFFFFFF).any() == any_hit

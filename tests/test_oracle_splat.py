@@ -172,3 +172,165 @@ def test_backward_matches_finite_differences():
             m[i, c] -= eps
             fd = (loss(p) - loss(m)) / (2 * eps)
             assert abs(fd - grad[i, c]) <= 2e-3 * max(1.0, abs(fd)), (i, c, fd, grad[i, c])
+
+
+# ---- the point-sprite footprint ---------------------------------------------
+# splat.vert.glsl:37-49 sizes the sprite of a splat at window position
+# ((ndc + 1) * W/2, (ndc + 1) * H/2) as
+#   lod * (1.1 * H / clip.z) * max(max(|ndc_x * W/H|, |ndc_y|) + 0.03, 1)
+# (clip.z, not clip.w), clamped to [1, 2048]; a pixel is a fragment when its
+# centre c satisfies lo <= c < hi.  The cases below are chosen so that the
+# voxel's box covers the whole sprite window: the hit set is then the window,
+# and any change to the sizing rule or the coverage rule changes it.
+_NEAR, _FAR = 0.1, 4096.0
+
+
+def _view_centre(pos, loc, yaw, pitch, lod):
+    """Box centre pos + lod/2 in view space, float64: translate by -location,
+    rotate by yaw about y, then by pitch about x (game.c:458-491)."""
+    a = np.asarray(pos, np.float64) + 0.5 * lod - np.asarray(loc, np.float64)
+    cy, sy = np.cos(np.radians(yaw)), np.sin(np.radians(yaw))
+    cp, sp = np.cos(np.radians(pitch)), np.sin(np.radians(pitch))
+    ry = np.array([[cy, 0, sy], [0, 1, 0], [-sy, 0, cy]])
+    rx = np.array([[1, 0, 0], [0, cp, -sp], [0, sp, cp]])
+    return rx @ (ry @ a)
+
+
+def _sprite_window(pos, loc, yaw, pitch, fov, W, H, lod=1):
+    """(lo_x, hi_x, lo_y, hi_y, base_side) from the perspective projection's
+    definition, in float64."""
+    v = _view_centre(pos, loc, yaw, pitch, lod)
+    f = 1.0 / np.tan(np.radians(fov) / 2)
+    w = -v[2]
+    clip_x, clip_y = f / (W / H) * v[0], f * v[1]
+    clip_z = (_FAR + _NEAR) / (_NEAR - _FAR) * v[2] + 2 * _FAR * _NEAR / (_NEAR - _FAR)
+    nx, ny = clip_x / w, clip_y / w
+    assert abs(nx) < 1 and abs(ny) < 1 and -w < clip_z < w     # not clipped
+    base = 1.1 * H / clip_z
+    side = lod * base * max(max(abs(nx * W / H), abs(ny)) + 0.03, 1.0)
+    side = min(max(side, 1.0), 2048.0)
+    wx, wy = (nx + 1) * W / 2, (ny + 1) * H / 2
+    return wx - side / 2, wx + side / 2, wy - side / 2, wy + side / 2, base * lod
+
+
+def _covered(lo, hi, n):
+    c = np.arange(n) + 0.5
+    return (lo <= c) & (c < hi)
+
+
+def _edge_margin(lo, hi, n):
+    c = np.arange(n) + 0.5
+    return min(np.abs(c - lo).min(), np.abs(c - hi).min())
+
+
+def _box_entry(cam, pos, loc, lod):
+    """Per pixel (t0, t1) of the ray through the pixel centre against the box,
+    float64 slab test on the float32 rays.  Rays and box are in world axes
+    with the camera at the origin (splat.frag.glsl:48-58 unprojects through
+    the inverse of projection * view_rotation)."""
+    dx, dy, dz = (np.asarray(d, np.float64) for d in _np_rays(cam))
+    v = np.asarray(pos, np.float64) + 0.5 * lod - np.asarray(loc, np.float64)
+    t0 = np.full(dx.shape, -np.inf)
+    t1 = np.full(dx.shape, np.inf)
+    for k, d in enumerate((dx, dy, dz)):
+        with np.errstate(divide="ignore", invalid="ignore"):
+            ta, tb = (v[k] - lod / 2) / d, (v[k] + lod / 2) / d
+        t0 = np.maximum(t0, np.minimum(ta, tb))
+        t1 = np.minimum(t1, np.maximum(ta, tb))
+    return t0, t1
+
+
+# (pos, view-space offset of the box centre (x, y, distance), yaw, pitch, fov,
+# W, H): head-on, off-axis, oblique (looking down a diagonal), and at
+# ndc_x = 0.75 on a 2:1 screen, where |ndc_x * W/H| + 0.03 > 1 enlarges the
+# sprite
+_SPRITE_CASES = [
+    ((10, 20, 30), (0.0, 0.0, 2.5), 0.0, 0.0, 30.0, 96, 64),
+    ((10, 20, 30), (-0.7, 0.35, 2.35), 0.0, 0.0, 30.0, 96, 64),
+    ((40, 30, 40), (0.4, -0.3, 2.2), 45.0, 35.0, 30.0, 80, 64),
+    ((100, 50, 60), (None, 0.3, 5.6), 0.0, 0.0, 20.0, 128, 64),
+]
+
+
+def _case(i):
+    """The case with the camera location that puts the box centre at the
+    given view-space offset."""
+    pos, (ox, oy, dist), yaw, pitch, fov, W, H = _SPRITE_CASES[i]
+    if ox is None:
+        ox = 0.75 * dist * (W / H) * np.tan(np.radians(fov) / 2)
+    cy, sy = np.cos(np.radians(yaw)), np.sin(np.radians(yaw))
+    cp, sp = np.cos(np.radians(pitch)), np.sin(np.radians(pitch))
+    rot = (np.array([[1, 0, 0], [0, cp, -sp], [0, sp, cp]]) @
+           np.array([[cy, 0, sy], [0, 1, 0], [-sy, 0, cy]]))
+    loc = np.asarray(pos, np.float64) + 0.5 - rot.T @ np.array([ox, oy, -dist])
+    return pos, tuple(float(v) for v in loc), yaw, pitch, fov, W, H
+
+
+@pytest.mark.parametrize("case", range(len(_SPRITE_CASES)))
+def test_sprite_window_is_the_footprint(case):
+    pos, loc, yaw, pitch, fov, W, H = _case(case)
+    cam = make_camera(loc, yaw, pitch, W, H, fov_deg=fov)
+    lo_x, hi_x, lo_y, hi_y, base = _sprite_window(pos, loc, yaw, pitch, fov, W, H)
+    # no pixel centre within 0.05 px of an edge: float32 cannot move it across
+    assert _edge_margin(lo_x, hi_x, W) > 0.05 and _edge_margin(lo_y, hi_y, H) > 0.05
+    want = _covered(lo_y, hi_y, H)[:, None] & _covered(lo_x, hi_x, W)[None, :]
+    # the box covers the window with room to spare, and reaches beyond it
+    t0, t1 = _box_entry(cam, pos, loc, 1)
+    inside = t1 - t0 > 1e-3
+    assert inside[want].all() and (inside & ~want).any()
+    splats = np.array([_rec(*pos, 0b101101)], np.int16)
+    depth, ids, _, _ = O.splat_render(splats, cam)
+    got = ids != 0xFFFFFFFF
+    assert np.array_equal(got, want), (np.argwhere(got ^ want)[:8], (lo_x, hi_x, lo_y, hi_y))
+    # depth = box entry distance / far
+    assert np.allclose(depth[got], t0[got] / _FAR, rtol=1e-5, atol=0)
+    if case == 3:   # the edge factor is in effect: the side is 1.53 times the base size
+        assert hi_x - lo_x > 1.5 * base
+
+
+def test_sprite_window_is_half_open():
+    """A far voxel on the view axis: the side clamps to 1 pixel around the
+    window centre (32, 32), so lo = 31.5 and hi = 32.5 fall exactly on pixel
+    centres; only pixel 31 (lo <= c) is covered, pixel 32 (c = hi) is not."""
+    W = H = 64
+    pos = (7, 9, 11)
+    loc = (7.5, 9.5, 111.5)
+    cam = make_camera(loc, 0.0, 0.0, W, H, fov_deg=10.0)
+    lo_x, hi_x, lo_y, hi_y, base = _sprite_window(pos, loc, 0.0, 0.0, 10.0, W, H)
+    assert base < 1.0 and (lo_x, hi_x, lo_y, hi_y) == (31.5, 32.5, 31.5, 32.5)
+    t0, t1 = _box_entry(cam, pos, loc, 1)
+    assert (t1 - t0 > 1e-3)[31:33, 31:33].all()   # the box would cover both
+    depth, ids, _, _ = O.splat_render(np.array([_rec(*pos, 7)], np.int16), cam)
+    assert np.argwhere(ids != 0xFFFFFFFF).tolist() == [[31, 31]]
+
+
+def test_far_plane_limits_the_entry_distance():
+    """A fragment is kept only if t0 / far < 1 (GL_LESS against the depth
+    buffer cleared to 1.0).  Two 256-voxel splats off the axis, both with
+    centres inside the view volume: the one whose box entry lies beyond far
+    on every covered ray leaves no fragment, the nearer one does."""
+    W = H = 64
+    loc = (2000.5, 300.5, 8000.5)
+    lod_log2, lod = 8, 256
+    f = np.tan(np.radians(25.0))
+    out = {}
+    for name, zc in (("beyond", 4000.0), ("inside", 3600.0)):
+        # box centre at view (0.466 z, 0, -z): 25 degrees off the axis
+        centre = np.array([loc[0] + f * zc, loc[1], loc[2] - zc])
+        pos = tuple(int(round(c - lod / 2)) for c in centre)
+        cam = make_camera(loc, 0.0, 0.0, W, H)
+        lo_x, hi_x, lo_y, hi_y, _ = _sprite_window(pos, loc, 0.0, 0.0, 70.0, W, H, lod)
+        want = _covered(lo_y, hi_y, H)[:, None] & _covered(lo_x, hi_x, W)[None, :]
+        t0, t1 = _box_entry(cam, pos, loc, lod)
+        hits = want & (t1 - t0 > 1e-3)
+        assert hits.sum() >= 4                     # rays of the window reach the box
+        depth, ids, _, _ = O.splat_render(np.array([_rec(*pos, 9)], np.int16), cam,
+                                          lod_log2=lod_log2)
+        out[name] = (hits, t0, depth, ids)
+    hits, t0, depth, ids = out["beyond"]
+    assert t0[hits].min() > 1.02 * _FAR and (ids == 0xFFFFFFFF).all() and (depth == 1.0).all()
+    hits, t0, depth, ids = out["inside"]
+    assert t0[hits].max() < 0.98 * _FAR
+    got = ids != 0xFFFFFFFF
+    assert np.array_equal(got, hits)
+    assert np.allclose(depth[got], t0[got] / _FAR, rtol=1e-5, atol=0)
