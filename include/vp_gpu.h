@@ -7,6 +7,16 @@
  * buffers belong to the caller; a vp_ctx owns only its small scratch arenas.
  * Output order is deterministic: it never depends on scheduling.
  *
+ * Streams: the calls made on one vp_ctx share its scratch (the work-queue
+ * ticket and look-back words, the splat frame and ray buffers, the block
+ * lists, the status word), so a context must be used from one stream at a
+ * time: calls queued on two streams at once may overwrite each other's
+ * scratch.  Use one context per stream (as the reference uses one set of
+ * buffers per meshing thread, src/chunkset.c:258-273).  A remesh graph
+ * (vp_host_remesh_graph_create) owns its own ticket and look-back words and
+ * may replay beside calls on the context; it shares only the status word,
+ * whose bits are sticky flags.
+ *
  * Reference interfaces replaced (paths are relative to kosshi-net/voxplat):
  *   vp_gpu_cull_mesh  <- chunkset_manage per-chunk body   src/chunkset.c:318-505
  *                        = chunk_make_mask                src/chunkset/mesher.c:377-456

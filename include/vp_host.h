@@ -40,7 +40,13 @@ int vp_host_remesh_per_chunk(vp_ctx *ctx, const vp_world_desc *world,
 
 /* The batched path captured once in a hipGraph and replayed per frame.  The
  * arguments are those of vp_gpu_cull_mesh; with d_n set, each replay remeshes
- * the first *d_n entries of d_chunk_ids (the dirty count lives on the GPU). */
+ * the first *d_n entries of d_chunk_ids (the dirty count lives on the GPU).
+ * Ownership: the graph allocates its own work-queue ticket and n look-back
+ * words at creation and frees them in _destroy, so later calls on the
+ * context (which may grow and replace the context's scratch) cannot leave it
+ * pointing at freed memory.  The buffers the caller passes (voxels, ids,
+ * d_n, outputs) must outlive the graph.  Replays of one graph run one at a
+ * time (they share its scratch). */
 typedef struct vp_remesh_graph vp_remesh_graph;
 int vp_host_remesh_graph_create(vp_ctx *ctx, const vp_world_desc *world,
                                 const uint8_t *d_voxels, const int16_t *d_uniform,

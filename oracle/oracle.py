@@ -347,7 +347,9 @@ def octree_cull(w: WorldDesc, planes, origin, node_items, chunk_mesh=None):
             return
         d = [f(o[i] - c[i]) for i in range(3)]
         dist = f(np.sqrt(f(f(f(d[0] * d[0]) + f(d[1] * d[1])) + f(d[2] * d[2]))))
-        if dist > f(lod * 1024):
+        # level 0 is a leaf: with the camera on a chunk centre dist > 0 fails and
+        # the reference's uint8 lod - 1 (game.c:386) would walk level 255
+        if dist > f(lod * 1024) or lod == 0:
             if lod == 0 and chunk_mesh is not None:
                 chunk = ((z << w.max_bitw[1] | y) << w.max_bitw[0]) | x
                 if chunk_mesh[chunk]:

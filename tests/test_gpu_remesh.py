@@ -234,3 +234,51 @@ def test_output_past_4gib(gpu):
     assert torch.equal(recs[2 * n1:3 * n1], first)
     del recs
     torch.cuda.empty_cache()
+
+
+def test_graph_keeps_its_scratch_when_the_context_grows(gpu):
+    """A captured remesh graph owns its ticket and look-back words: calls on
+    the same context that grow (and so replace) the context's scratch do not
+    touch what the graph's replays use.  The graph is replayed after an RLE
+    encode and a remesh whose lists need more look-back words than the
+    context started with, and after both run beside it, on the same world."""
+    import torch
+
+    from voxplat_amd import GPU, RemeshGraph
+
+    cfg = W.small_world(6, (2, 1, 2), 0x5EED0101, hmax=100)
+    w = cfg.desc()
+    vox = O.gen_world(w, cfg.gen())
+    smap = O.shadow_build(w, vox)
+    ids = np.random.default_rng(5).permutation(w.chunks).astype(np.uint32)
+    erecs, eoffs, ecounts = O.remesh_list(w, vox, ids, smap, threads=8)
+    g2 = GPU(0)                  # a fresh context: 4096 look-back words to start with
+    try:
+        vd, sd, idd = dev(vox), dev(smap), ids_dev(ids)
+        dn = dev(np.array([len(ids)], np.uint32))
+        total = len(erecs)
+        recs = torch.empty(total * 4, dtype=torch.int16, device="cuda")
+        offs = torch.empty(len(ids) + 1, dtype=torch.int64, device="cuda")
+        counts = torch.empty(len(ids) * 5, dtype=torch.int32, device="cuda")
+        graph = RemeshGraph(g2, w, vd, idd, sd, offs, counts, recs, total, d_n=dn)
+
+        def replay_and_check():
+            recs.fill_(0)
+            graph.launch()
+            torch.cuda.synchronize()
+            assert g2.status() == 0
+            assert np.array_equal(host(offs, np.uint64), eoffs)
+            assert np.array_equal(host(counts, np.uint32).reshape(-1, 5), ecounts)
+            assert np.array_equal(host(recs).reshape(-1, 4), erecs)
+
+        replay_and_check()
+        big = ids_dev(np.tile(ids, 20000 // len(ids) + 1)[:20000])   # 20,000 entries
+        g2.rle_encode(w, vd, big)
+        torch.cuda.synchronize()
+        replay_and_check()
+        got = gpu_remesh(g2, w, vd, np.tile(ids, 600)[:9000], sd)
+        assert got[3] == 0 and len(got[0]) > total
+        replay_and_check()
+        graph.close()
+    finally:
+        g2.close()

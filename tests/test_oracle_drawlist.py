@@ -206,3 +206,22 @@ def test_octree_host_matches_oracle(seed):
         assert [tuple(int(v) for v in d) for d in draws] == [tuple(d) for d in edraws]
         assert list(mesh) == emesh and tot == etot
     assert len(edraws) > 0
+
+
+def test_octree_camera_on_a_chunk_centre():
+    """The camera exactly on a chunk centre: the level-0 node there has
+    dist = 0, which is not > 0 * 1024; it is still a leaf and is drawn (the
+    reference would recurse into level 255, game.c:386)."""
+    w = abi.WorldDesc.make(6, (5, 2, 5))
+    rng = np.random.default_rng(11)
+    items = _node_items(w, rng)
+    items[0][:] = np.maximum(items[0], 1)
+    loc = (5 * 64 + 32.0, 1 * 64 + 32.0, 7 * 64 + 32.0)
+    vp = W.make_view_projection(loc, 30.0, 20.0, 1920, 1080)
+    planes = frustum_planes(vp)
+    edraws, _, etot = O.octree_cull(w, planes, loc, items)
+    draws, _, tot = octree_cull(w, planes, loc, _offsets(items))
+    assert [tuple(int(v) for v in d) for d in draws] == [tuple(d) for d in edraws]
+    assert tot == etot
+    here = (7 << 7 | 1 << 5) | 5
+    assert (0, here) in [(lvl, node) for lvl, node, _ in edraws]

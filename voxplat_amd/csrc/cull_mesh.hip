@@ -1054,20 +1054,25 @@ static int check_world(const vp_world_desc *w)
 
 using namespace vp;
 
-extern "C" int vp_gpu_cull_mesh(vp_ctx *ctx, const vp_world_desc *world,
-                                const uint8_t *d_voxels, const int16_t *d_uniform,
-                                const vp_shadow_desc *shadow,
-                                const uint32_t *d_chunk_ids, uint32_t n, const uint32_t *d_n,
-                                uint32_t flags, const vp_splat_out *out, void *stream)
+namespace vp {
+
+int cull_mesh_check(const vp_world_desc *world, const uint8_t *d_voxels,
+                    const uint32_t *d_chunk_ids, uint32_t flags, const vp_splat_out *out)
 {
-    if (!ctx || !d_voxels || !d_chunk_ids || !out || !out->d_chunk_offsets ||
-        !out->d_level_counts)
+    if (!d_voxels || !d_chunk_ids || !out || !out->d_chunk_offsets || !out->d_level_counts)
         return VP_ERR_ARG;
     int rc = check_world(world);
     if (rc) return rc;
     if (!(flags & VP_CM_COUNT_ONLY) && !out->d_splats && out->capacity_records) return VP_ERR_ARG;
-    if (n == 0) return VP_OK;
-    hipStream_t st = as_stream(stream);
+    return VP_OK;
+}
+
+int cull_mesh_launch(vp_ctx *ctx, const vp_world_desc *world, const uint8_t *d_voxels,
+                     const int16_t *d_uniform, const vp_shadow_desc *shadow,
+                     const uint32_t *d_chunk_ids, uint32_t n, const uint32_t *d_n,
+                     uint32_t flags, const vp_splat_out *out, uint32_t *ticket,
+                     uint64_t *state, hipStream_t st)
+{
     CullArgs a = {};
     a.voxels = d_voxels;
     a.uniform = d_uniform;
@@ -1090,23 +1095,44 @@ extern "C" int vp_gpu_cull_mesh(vp_ctx *ctx, const vp_world_desc *world,
     // one 64^3 chunk alone: a 1024-thread workgroup (16 waves on one CU)
     // finishes it in 0.70x the time of the batched path's 512-thread shape
     if (a.single && world->root_bitw == 6) bs = 1024;
-    {
-        std::lock_guard<std::mutex> g(ctx->mu);
-        if (!a.single) {
-            rc = ensure_state(ctx, n);
-            if (rc) return rc;
-            a.ticket = ctx->d_ticket;
-            a.state = ctx->d_state;
-            const int resident = ctx->num_cus * ctx->cull_wg_per_cu;
-            grid = (int)((uint64_t)n < (uint64_t)resident ? n : (uint32_t)resident);
-            const int rgrid = (int)((n + 255u) / 256u);
-            hipLaunchKernelGGL(k_cull_reset, dim3(rgrid < 1024 ? rgrid : 1024), dim3(256), 0, st,
-                               a.ticket, a.state, n, a.offsets);
-            VP_HIP_TRY(hipGetLastError());
-        }
+    if (!a.single) {
+        if (!ticket || !state) return VP_ERR_ARG;
+        a.ticket = ticket;
+        a.state = state;
+        const int resident = ctx->num_cus * ctx->cull_wg_per_cu;
+        grid = (int)((uint64_t)n < (uint64_t)resident ? n : (uint32_t)resident);
+        const int rgrid = (int)((n + 255u) / 256u);
+        hipLaunchKernelGGL(k_cull_reset, dim3(rgrid < 1024 ? rgrid : 1024), dim3(256), 0, st,
+                           a.ticket, a.state, n, a.offsets);
+        VP_HIP_TRY(hipGetLastError());
     }
     VP_HIP_TRY(dispatch<false>(world->root_bitw, a, (flags & VP_CM_RECORD_V2) != 0, bs, grid, st));
     return VP_OK;
+}
+
+}  // namespace vp
+
+extern "C" int vp_gpu_cull_mesh(vp_ctx *ctx, const vp_world_desc *world,
+                                const uint8_t *d_voxels, const int16_t *d_uniform,
+                                const vp_shadow_desc *shadow,
+                                const uint32_t *d_chunk_ids, uint32_t n, const uint32_t *d_n,
+                                uint32_t flags, const vp_splat_out *out, void *stream)
+{
+    if (!ctx) return VP_ERR_ARG;
+    int rc = cull_mesh_check(world, d_voxels, d_chunk_ids, flags, out);
+    if (rc) return rc;
+    if (n == 0) return VP_OK;
+    hipStream_t st = as_stream(stream);
+    if (n == 1 && !d_n)   // the plain per-chunk path needs no look-back state
+        return cull_mesh_launch(ctx, world, d_voxels, d_uniform, shadow, d_chunk_ids, n, d_n,
+                                flags, out, nullptr, nullptr, st);
+    // the context's look-back state is grown to n words under the lock and
+    // used by the launch; calls on one context run one stream at a time
+    std::lock_guard<std::mutex> g(ctx->mu);
+    rc = ensure_state(ctx, n);
+    if (rc) return rc;
+    return cull_mesh_launch(ctx, world, d_voxels, d_uniform, shadow, d_chunk_ids, n, d_n, flags,
+                            out, ctx->d_ticket, ctx->d_state, st);
 }
 
 extern "C" int vp_gpu_make_mask(vp_ctx *ctx, const vp_world_desc *world,

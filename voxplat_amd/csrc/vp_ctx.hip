@@ -25,19 +25,29 @@ int ensure_state(vp_ctx *ctx, uint64_t words)
 
 int ensure_frame(vp_ctx *ctx, uint64_t pixels)
 {
-    if (pixels <= ctx->frame_cap) return VP_OK;
+    if (pixels <= ctx->frame_cap && pixels <= ctx->rays_cap) return VP_OK;
+    // both buffers are allocated before either capacity changes, so a failed
+    // allocation leaves the old (smaller) pair and its capacities in place
     uint64_t *p = nullptr;
+    float4 *r = nullptr;
     if (hipMalloc(&p, pixels * sizeof(uint64_t)) != hipSuccess) return VP_ERR_NOMEM;
-    if (ctx->d_frame) {
-        if (hipDeviceSynchronize() != hipSuccess) return VP_ERR_HIP;
-        (void)hipFree(ctx->d_frame);
+    if (hipMalloc(&r, pixels * sizeof(float4)) != hipSuccess) {
+        (void)hipFree(p);
+        return VP_ERR_NOMEM;
+    }
+    if (ctx->d_frame || ctx->d_rays) {
+        // the old buffers may still be in use by queued work
+        if (hipDeviceSynchronize() != hipSuccess) {
+            (void)hipFree(p);
+            (void)hipFree(r);
+            return VP_ERR_HIP;
+        }
+        if (ctx->d_frame) (void)hipFree(ctx->d_frame);
+        if (ctx->d_rays) (void)hipFree(ctx->d_rays);
     }
     ctx->d_frame = p;
-    ctx->frame_cap = pixels;
-    float4 *r = nullptr;
-    if (hipMalloc(&r, pixels * sizeof(float4)) != hipSuccess) return VP_ERR_NOMEM;
-    if (ctx->d_rays) (void)hipFree(ctx->d_rays);
     ctx->d_rays = r;
+    ctx->frame_cap = pixels;
     ctx->rays_cap = pixels;
     return VP_OK;
 }

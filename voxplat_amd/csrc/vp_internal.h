@@ -46,6 +46,18 @@ int ensure_cmap(vp_ctx *ctx, uint64_t chunks);
 
 inline hipStream_t as_stream(void *s) { return reinterpret_cast<hipStream_t>(s); }
 
+// vp_gpu_cull_mesh's argument checks, and its launch with explicit scratch:
+// a ticket word and n look-back words (both unused when n == 1 and d_n is
+// NULL).  The context's own scratch, a graph's private scratch and the
+// static scratch of the allocation-free entry all go through it.
+int cull_mesh_check(const vp_world_desc *world, const uint8_t *d_voxels,
+                    const uint32_t *d_chunk_ids, uint32_t flags, const vp_splat_out *out);
+int cull_mesh_launch(vp_ctx *ctx, const vp_world_desc *world, const uint8_t *d_voxels,
+                     const int16_t *d_uniform, const vp_shadow_desc *shadow,
+                     const uint32_t *d_chunk_ids, uint32_t n, const uint32_t *d_n,
+                     uint32_t flags, const vp_splat_out *out, uint32_t *ticket,
+                     uint64_t *state, hipStream_t st);
+
 // relaxed device-scope atomics on 64-bit words (coherent across XCDs)
 __device__ __forceinline__ uint64_t ld_relaxed(const uint64_t *p)
 {

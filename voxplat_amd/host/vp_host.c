@@ -1,7 +1,7 @@
 /*
  * vp_host.c -- host-side C of the pipeline: the order-dependent shadow-map
  * updates of sphere edits, the remesh drivers (plain per-chunk loop,
- * captured batched frame, launch-shape autotuner) and the per-frame draw
+ * launch-shape autotuner) and the per-frame draw
  * queue (frustum planes, octree LOD selection) declared in vp_host.h.
  *
  * shadow_place_update (src/chunkset/shadow.h:77-89) reads map[i] and map[i+1]
@@ -69,8 +69,8 @@ int vp_host_shadow_spheres(const vp_world_desc *world, uint16_t *map,
 }
 
 /* ------------------------------------------------------------------------
- * Remesh drivers: the plain per-chunk loop, the captured batched frame and
- * the launch-shape autotuner.
+ * Remesh drivers: the plain per-chunk loop and the launch-shape autotuner
+ * (the captured batched frame is in csrc/remesh_graph.hip).
  * ---------------------------------------------------------------------- */
 #include <stdlib.h>
 #include <string.h>
@@ -98,64 +98,6 @@ int vp_host_remesh_per_chunk(vp_ctx *ctx, const vp_world_desc *world,
                                   NULL, flags, &o, stream);
         if (rc) return rc;
     }
-    return VP_OK;
-}
-
-struct vp_remesh_graph {
-    hipGraph_t graph;
-    hipGraphExec_t exec;
-};
-
-int vp_host_remesh_graph_create(vp_ctx *ctx, const vp_world_desc *world,
-                                const uint8_t *d_voxels, const int16_t *d_uniform,
-                                const vp_shadow_desc *shadow, const uint32_t *d_chunk_ids,
-                                uint32_t n, const uint32_t *d_n, uint32_t flags,
-                                const vp_splat_out *out, vp_remesh_graph **graph)
-{
-    if (!ctx || !graph) return VP_ERR_ARG;
-    *graph = NULL;
-    int rc = vp_ctx_reserve(ctx, n, 0);
-    if (rc) return rc;
-    hipStream_t s;
-    if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) return VP_ERR_HIP;
-    vp_remesh_graph *g = (vp_remesh_graph *)calloc(1, sizeof(*g));
-    if (!g) {
-        (void)hipStreamDestroy(s);
-        return VP_ERR_NOMEM;
-    }
-    rc = VP_ERR_HIP;
-    if (hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) == hipSuccess) {
-        int crc = vp_gpu_cull_mesh(ctx, world, d_voxels, d_uniform, shadow, d_chunk_ids, n, d_n,
-                                   flags, out, s);
-        hipError_t e = hipStreamEndCapture(s, &g->graph);
-        if (crc != VP_OK)
-            rc = crc;
-        else if (e == hipSuccess &&
-                 hipGraphInstantiate(&g->exec, g->graph, NULL, NULL, 0) == hipSuccess)
-            rc = VP_OK;
-    }
-    (void)hipStreamDestroy(s);
-    if (rc != VP_OK) {
-        if (g->graph) (void)hipGraphDestroy(g->graph);
-        free(g);
-        return rc;
-    }
-    *graph = g;
-    return VP_OK;
-}
-
-int vp_host_remesh_graph_launch(vp_remesh_graph *graph, void *stream)
-{
-    if (!graph) return VP_ERR_ARG;
-    return hipGraphLaunch(graph->exec, (hipStream_t)stream) == hipSuccess ? VP_OK : VP_ERR_HIP;
-}
-
-int vp_host_remesh_graph_destroy(vp_remesh_graph *graph)
-{
-    if (!graph) return VP_ERR_ARG;
-    (void)hipGraphExecDestroy(graph->exec);
-    (void)hipGraphDestroy(graph->graph);
-    free(graph);
     return VP_OK;
 }
 
@@ -276,7 +218,10 @@ static void cull_node(cull_state *s, uint32_t lod, uint32_t x, uint32_t y, uint3
     if (!sphere_visible(s->planes, c, (float)((double)ws_root * 0.85))) return;
     const float d0 = s->origin[0] - c[0], d1 = s->origin[1] - c[1], d2 = s->origin[2] - c[2];
     const float dist = sqrtf((d0 * d0 + d1 * d1) + d2 * d2);
-    if (dist > (float)(lod * 1024u)) {
+    /* A level-0 node is always a leaf.  The reference would visit level
+     * 255 (uint8 lod - 1, game.c:386) when the camera sits on a chunk
+     * centre, where dist > 0 * 1024 fails. */
+    if (dist > (float)(lod * 1024u) || lod == 0) {
         if (lod == 0 && s->chunk_mesh) {
             const uint32_t chunk = ((z << w->max_bitw[1] | y) << w->max_bitw[0]) | x;
             if (s->chunk_mesh[chunk]) {
