@@ -52,7 +52,7 @@ def lib() -> C.CDLL:
         "vpo_rle_decompress": (C.c_int64, [_P, C.c_uint32, _P, C.c_uint32]),
         "vpo_edit_sphere": (None, [W, _P, _P, C.c_uint32, _P, C.c_uint32, C.c_uint8, _P]),
         "vpo_splat_render": (None, [_P, _P, C.c_uint32, C.c_uint64, C.POINTER(Camera), _P, _P,
-                                    _P, _P, _P]),
+                                    _P, _P, _P, C.c_int]),
         "vpo_splat_backward": (None, [C.c_uint64, C.POINTER(Camera), _P, _P, _P, _P]),
         "vpo_chunk_make_mesh": (C.c_uint32, [W, _P, C.c_uint32, _P, C.c_uint32, _P, _P]),
         "vpo_mesh_list": (C.c_int64, [W, _P, _P, C.c_uint32, _P, C.c_uint32, C.c_int, _P, _P,
@@ -230,7 +230,9 @@ def edit_sphere(w: WorldDesc, voxels: np.ndarray, smap: np.ndarray | None, centr
 
 def splat_render(splats: np.ndarray, cam: Camera, lod: np.ndarray | None = None,
                  lod_log2: int = 0, voxel_rgb: np.ndarray | None = None, rgb: bool = False,
-                 weight: bool = False):
+                 weight: bool = False, threads: int = 1):
+    """threads > 1 splits the splats over OpenMP threads; the result is the
+    same for every thread count."""
     splats = np.ascontiguousarray(splats, np.int16).reshape(-1, 4)
     n = len(splats)
     H, W = cam.height, cam.width
@@ -243,7 +245,7 @@ def splat_render(splats: np.ndarray, cam: Camera, lod: np.ndarray | None = None,
     if lod is not None:
         lod = np.ascontiguousarray(lod, np.uint8)
     lib().vpo_splat_render(_p(splats), _p(lod), lod_log2, n, C.byref(cam), _p(voxel_rgb),
-                           _p(depth), _p(ids), _p(col), _p(wt))
+                           _p(depth), _p(ids), _p(col), _p(wt), threads)
     return depth, ids, col, wt
 
 

@@ -799,22 +799,13 @@ static inline float u2f(uint32_t u) { float f; memcpy(&f, &u, 4); return f; }
 
 static float glsl_sign(float x) { return x > 0.0f ? 1.0f : (x < 0.0f ? -1.0f : 0.0f); }
 
-void vpo_splat_render(const int16_t *splats, const uint8_t *lod, uint32_t lod_log2,
-                      uint64_t n, const vp_camera *cam, const float *voxel_rgb,
-                      float *depth, uint32_t *id, float *rgb, float *weight)
+/* Rasterise splats [i0, i1) into best[] (packed depth|id keys, min wins). */
+static void splat_range(const int16_t *splats, const uint8_t *lod, uint32_t lod_log2,
+                        uint64_t i0, uint64_t i1, const vp_camera *cam, const float *rays,
+                        uint64_t *best)
 {
-    uint32_t W = cam->width, H = cam->height;
-    uint64_t np = (uint64_t)W * H;
-    float *rays = (float *)malloc(sizeof(float) * 6 * np);
-    uint64_t *best = (uint64_t *)malloc(sizeof(uint64_t) * np);
-    for (uint32_t py = 0; py < H; py++)
-        for (uint32_t px = 0; px < W; px++) {
-            float *r = rays + 6 * ((uint64_t)py * W + px);
-            pixel_ray(cam, (int)px, (int)py, r);
-            r[3] = 1.0f / r[0]; r[4] = 1.0f / r[1]; r[5] = 1.0f / r[2];
-        }
-    for (uint64_t p = 0; p < np; p++) best[p] = ~0ull;
-    for (uint64_t i = 0; i < n; i++) {
+    uint32_t W = cam->width;
+    for (uint64_t i = i0; i < i1; i++) {
         float lodf = (float)(1u << (lod ? lod[i] : lod_log2));
         splat_vs vs;
         if (!splat_vertex(splats + 4 * i, lodf, cam, &vs)) continue;
@@ -837,7 +828,51 @@ void vpo_splat_render(const int16_t *splats, const uint8_t *lod, uint32_t lod_lo
             }
         }
     }
+}
+
+/* threads > 1: each thread rasterises a contiguous range of splats into a
+ * private frame and the frames are merged by the minimum key.  The key
+ * (depth bits, index) orders hits exactly as the serial loop's depth test
+ * with ties to the lower index does, so the result does not depend on the
+ * thread count. */
+void vpo_splat_render(const int16_t *splats, const uint8_t *lod, uint32_t lod_log2,
+                      uint64_t n, const vp_camera *cam, const float *voxel_rgb,
+                      float *depth, uint32_t *id, float *rgb, float *weight, int threads)
+{
+    uint32_t W = cam->width, H = cam->height;
+    uint64_t np = (uint64_t)W * H;
+    float *rays = (float *)malloc(sizeof(float) * 6 * np);
+    if (threads < 1) threads = 1;
+    if ((uint64_t)threads > n / 4096 + 1) threads = (int)(n / 4096 + 1);
+    uint64_t *frames = (uint64_t *)malloc(sizeof(uint64_t) * np * (uint64_t)threads);
+    uint64_t *best = frames;
+#pragma omp parallel for num_threads(threads) schedule(static)
+    for (uint32_t py = 0; py < H; py++)
+        for (uint32_t px = 0; px < W; px++) {
+            float *r = rays + 6 * ((uint64_t)py * W + px);
+            pixel_ray(cam, (int)px, (int)py, r);
+            r[3] = 1.0f / r[0]; r[4] = 1.0f / r[1]; r[5] = 1.0f / r[2];
+        }
+#pragma omp parallel num_threads(threads)
+    {
+        int t = omp_get_thread_num(), nt = omp_get_num_threads();
+        uint64_t *mine = frames + np * (uint64_t)t;
+        for (uint64_t p = 0; p < np; p++) mine[p] = ~0ull;
+        uint64_t i0 = n * (uint64_t)t / (uint64_t)nt, i1 = n * (uint64_t)(t + 1) / (uint64_t)nt;
+        splat_range(splats, lod, lod_log2, i0, i1, cam, rays, mine);
+#pragma omp barrier
+#pragma omp for schedule(static)
+        for (uint64_t p = 0; p < np; p++) {
+            uint64_t k = frames[p];
+            for (int u = 1; u < nt; u++) {
+                uint64_t v = frames[np * (uint64_t)u + p];
+                if (v < k) k = v;
+            }
+            best[p] = k;
+        }
+    }
     const float light[3] = {-0.70710677f, 0.70710677f, 0.0f};  /* normalize(-1,1,0) */
+#pragma omp parallel for num_threads(threads) schedule(static)
     for (uint64_t p = 0; p < np; p++) {
         uint64_t k = best[p];
         if (k == ~0ull) {
@@ -896,7 +931,7 @@ void vpo_splat_render(const int16_t *splats, const uint8_t *lod, uint32_t lod_lo
         if (weight) weight[p] = (s1 * s2) * keep;
     }
     free(rays);
-    free(best);
+    free(frames);
 }
 
 void vpo_splat_backward(uint64_t n, const vp_camera *cam, const uint32_t *id,

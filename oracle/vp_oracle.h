@@ -104,7 +104,8 @@ void     vpo_edit_sphere(const vp_world_desc *w, uint8_t *voxels, uint16_t *map,
 /* ---- splat resolve (splat.vert.glsl / splat.frag.glsl) ------------------- */
 void     vpo_splat_render(const int16_t *splats, const uint8_t *lod, uint32_t lod_log2,
                           uint64_t n, const vp_camera *cam, const float *voxel_rgb,
-                          float *depth, uint32_t *id, float *rgb, float *weight);
+                          float *depth, uint32_t *id, float *rgb, float *weight,
+                          int threads);
 void     vpo_splat_backward(uint64_t n, const vp_camera *cam, const uint32_t *id,
                             const float *weight, const float *dL_drgb, double *grad);
 

@@ -334,3 +334,23 @@ def test_far_plane_limits_the_entry_distance():
     got = ids != 0xFFFFFFFF
     assert np.array_equal(got, hits)
     assert np.allclose(depth[got], t0[got] / _FAR, rtol=1e-5, atol=0)
+
+
+def test_threaded_resolve_equals_serial():
+    """Splat ranges on several threads, merged by the packed (depth, index)
+    minimum, give the serial loop's frame: ids, depth, colour and weight."""
+    rng = np.random.default_rng(21)
+    n = 60000
+    splats = np.zeros((n, 4), np.int16)
+    splats[:, 0] = rng.integers(0, 128, n)
+    splats[:, 1] = rng.integers(0, 24, n)
+    splats[:, 2] = rng.integers(0, 128, n)
+    splats[:, 3] = rng.integers(1, 128, n)
+    splats[n // 2:] = splats[:n // 2]                 # exact duplicates: ties by index
+    lod = rng.integers(0, 3, n).astype(np.uint8)
+    cam = make_camera((64.0, 70.0, 160.0), 10.0, 30.0, 160, 96)
+    a = O.splat_render(splats, cam, lod=lod, rgb=True, weight=True, threads=1)
+    b = O.splat_render(splats, cam, lod=lod, rgb=True, weight=True, threads=7)
+    assert (a[1] != 0xFFFFFFFF).sum() > 5000
+    for x, y in zip(a, b):
+        assert np.array_equal(x, y)
