@@ -95,7 +95,8 @@ def test_cfg4_gradient_matches_oracle_backward(gpu, cfg4):
         eg = O.splat_backward(n, cam, eids, ew, G)
         gg = host(x.grad)
         nz = np.nonzero(np.abs(eg).sum(1))[0]
-        assert len(nz) > 10000
+        # close-up views see a few thousand voxels, far ones tens of thousands
+        assert len(nz) > 1000
         assert np.allclose(gg, eg, rtol=1e-5, atol=1e-5)
 
 
