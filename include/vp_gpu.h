@@ -138,6 +138,24 @@ int vp_gpu_cull_mesh(vp_ctx *ctx, const vp_world_desc *world,
                      const uint32_t *d_chunk_ids, uint32_t n, const uint32_t *d_n,
                      uint32_t flags, const vp_splat_out *out, void *stream);
 
+/* vp_gpu_cull_mesh that allocates nothing: no counterpart in the reference.
+ * For callers that cannot allocate device memory (code under graph capture,
+ * tight serving loops).  Same arguments, output and errors as
+ * vp_gpu_cull_mesh, but the work-queue ticket and the n look-back words live
+ * in a static device array of each GPU (4 MiB + 256 B, held until the process
+ * ends) instead of the context's scratch, so n is limited to
+ * VP_NOALLOC_MAX_CHUNKS (every chunk of a 16384x512x16384 world of 64^3
+ * chunks); a larger n returns VP_ERR_CAPACITY and launches nothing.  All calls
+ * on one GPU share the array: each call clears what it uses before its
+ * kernel runs, and calls that overlap on one GPU (two streams, or two
+ * contexts of the same device) race on it. */
+#define VP_NOALLOC_MAX_CHUNKS 524288u
+int vp_gpu_cull_mesh_noalloc(vp_ctx *ctx, const vp_world_desc *world,
+                             const uint8_t *d_voxels, const int16_t *d_uniform,
+                             const vp_shadow_desc *shadow,
+                             const uint32_t *d_chunk_ids, uint32_t n, const uint32_t *d_n,
+                             uint32_t flags, const vp_splat_out *out, void *stream);
+
 /* ---- near-field quad mesh ---------------------------------------------------
  * Replaces chunk_make_mesh (src/chunkset/mesher.c:184-357), one call per
  * chunk in the reference, with one launch over a chunk list.

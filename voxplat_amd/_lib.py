@@ -116,11 +116,19 @@ class GPU:
     # ---- remesh --------------------------------------------------------------
     def cull_mesh(self, world: abi.WorldDesc, voxels, ids, shadow_map=None, uniform=None,
                   lod: bool = True, v2: bool = False, capacity: int | None = None,
-                  count_only: bool = False, d_n=None, out=None, records=None, stream=None):
+                  count_only: bool = False, d_n=None, out=None, records=None, stream=None,
+                  noalloc: bool = False):
         """Remesh chunks `ids` (int32 tensor). Returns (records int16[N,4] or None,
-        offsets int64[n+1] in int16 items, level_counts int32[n,5])."""
+        offsets int64[n+1] in int16 items, level_counts int32[n,5]).
+        noalloc: call vp_gpu_cull_mesh_noalloc (static scratch, n limited to
+        VP_NOALLOC_MAX_CHUNKS) instead of vp_gpu_cull_mesh."""
         torch = self.torch
         n = int(ids.numel())
+        if noalloc and n > abi.VP_NOALLOC_MAX_CHUNKS:
+            raise ValueError(f"vp_gpu_cull_mesh_noalloc takes at most "
+                             f"{abi.VP_NOALLOC_MAX_CHUNKS} chunks, got {n}")
+        entry = self.lib.vp_gpu_cull_mesh_noalloc if noalloc else self.lib.vp_gpu_cull_mesh
+        name = "vp_gpu_cull_mesh_noalloc" if noalloc else "vp_gpu_cull_mesh"
         dev = voxels.device
         _need_world(world, voxels, uniform, shadow_map)
         _need(ids, n * 4, "ids")
@@ -142,11 +150,11 @@ class GPU:
         so.d_chunk_offsets, so.d_level_counts = offsets.data_ptr(), counts.data_ptr()
         if count_only or capacity is None:
             so.d_splats, so.capacity_records = None, 0
-            _check(self.lib.vp_gpu_cull_mesh(
+            _check(entry(
                 self.ctx, C.byref(world), C.c_void_p(voxels.data_ptr()),
                 C.c_void_p(_ptr(uniform)), C.byref(sh), C.c_void_p(ids.data_ptr()), n,
                 C.c_void_p(_ptr(d_n)), flags | abi.VP_CM_COUNT_ONLY, C.byref(so), st),
-                "vp_gpu_cull_mesh(count)")
+                name + "(count)")
             if count_only:
                 return None, offsets, counts
             capacity = int(offsets[n].item()) // 4
@@ -154,10 +162,10 @@ class GPU:
             torch.empty((max(capacity, 1), 4), dtype=torch.int16, device=dev)
         _need(recs, capacity * 8, "records")
         so.d_splats, so.capacity_records = recs.data_ptr(), capacity
-        _check(self.lib.vp_gpu_cull_mesh(
+        _check(entry(
             self.ctx, C.byref(world), C.c_void_p(voxels.data_ptr()), C.c_void_p(_ptr(uniform)),
             C.byref(sh), C.c_void_p(ids.data_ptr()), n, C.c_void_p(_ptr(d_n)), flags,
-            C.byref(so), st), "vp_gpu_cull_mesh")
+            C.byref(so), st), name)
         return recs, offsets, counts
 
     def mesh(self, world: abi.WorldDesc, voxels, ids, shadow_map, uniform=None,

@@ -18,6 +18,7 @@ VP_STATUS_BAD_ID = 4
 VP_CM_LOD = 1
 VP_CM_RECORD_V2 = 2
 VP_CM_COUNT_ONLY = 4
+VP_NOALLOC_MAX_CHUNKS = 524288
 
 VP_RLE_ENCODE = 0
 VP_RLE_DECODE = 1
@@ -111,6 +112,10 @@ PROTOTYPES = {
     "vp_gpu_cull_mesh": (C.c_int, [C.c_void_p, C.POINTER(WorldDesc), C.c_void_p, C.c_void_p,
                                    C.POINTER(ShadowDesc), C.c_void_p, C.c_uint32, C.c_void_p,
                                    C.c_uint32, C.POINTER(SplatOut), C.c_void_p]),
+    "vp_gpu_cull_mesh_noalloc": (C.c_int, [C.c_void_p, C.POINTER(WorldDesc), C.c_void_p,
+                                           C.c_void_p, C.POINTER(ShadowDesc), C.c_void_p,
+                                           C.c_uint32, C.c_void_p, C.c_uint32,
+                                           C.POINTER(SplatOut), C.c_void_p]),
     "vp_gpu_mesh": (C.c_int, [C.c_void_p, C.POINTER(WorldDesc), C.c_void_p, C.c_void_p,
                               C.POINTER(ShadowDesc), C.c_void_p, C.c_uint32,
                               C.POINTER(MeshOut), C.c_void_p]),

@@ -111,6 +111,12 @@ extern "C" int vp_ctx_create(int device, vp_ctx **out)
         delete c;
         return VP_ERR_NOMEM;
     }
+    if (vp::noalloc_init(c) != VP_OK) {
+        (void)hipFree(c->d_status);
+        (void)hipFree(c->d_state);
+        delete c;
+        return VP_ERR_HIP;
+    }
     *out = c;
     return VP_OK;
 }

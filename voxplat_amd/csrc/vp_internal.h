@@ -32,6 +32,9 @@ struct vp_ctx {
     uint64_t  rays_cap = 0;
     uint32_t *d_cmap = nullptr;        // draw lists: list entry + 1 per chunk index
     uint64_t  cmap_cap = 0;
+    // device address of the static scratch of vp_gpu_cull_mesh_noalloc on
+    // this context's device (noalloc.hip); not owned
+    uint64_t *d_noalloc = nullptr;
     // tuned launch shape of the persistent remesh kernel
     int cull_block = 512;              // threads per workgroup
     int cull_wg_per_cu = 2;            // resident workgroups per CU
@@ -43,6 +46,8 @@ int ensure_state(vp_ctx *ctx, uint64_t words);
 int ensure_frame(vp_ctx *ctx, uint64_t pixels);
 int ensure_blocks(vp_ctx *ctx, uint64_t blocks);
 int ensure_cmap(vp_ctx *ctx, uint64_t chunks);
+// fetch the static scratch's device address on the current device
+int noalloc_init(vp_ctx *ctx);
 
 inline hipStream_t as_stream(void *s) { return reinterpret_cast<hipStream_t>(s); }
 
