@@ -1,20 +1,23 @@
 #!/usr/bin/env python3
 """Benchmark of the voxel pipeline on one MI355X.
 
-Workload (one step): a full remesh -- cull, level-0 splat list and LOD levels
-1..4 -- of every chunk of the 2048x256x2048 uint8 world (4096 chunks of 64^3,
-seeded value-noise terrain with 5% caves), as one batched persistent kernel
-captured in a hipGraph.  Inputs are resident in HBM before timing starts.
+Headline (one step): a full remesh -- cull, level-0 splat list and LOD levels
+1..4 -- of every chunk of the 16384x512x16384 uint8 world (524,288 chunks of
+64^3, 128 GiB resident in HBM, seeded value-noise terrain with 5% caves), as
+one batched persistent kernel captured in a hipGraph, the list ordered
+heaviest first.  Inputs are resident in HBM before timing starts.
 
-Also measured (not the headline value):
-  * the plain per-chunk path: one launch per chunk from the C host loop;
-  * splat resolve of the level-0 list at 1920x1080 over 16 seeded poses;
-  * the near-field quad mesh (chunk_make_mesh) of every chunk of the world;
-  * the octree draw lists: node buffers of all five levels, then per pose the
-    LOD selection, the assembled list and its splat resolve at 1920x1080;
-  * the roofline of the remesh kernel from HIP events around the timed steps;
-  * the CPU oracle (4 OpenMP threads, as chunkset_manage's team) on the same
-    4096 chunks.
+Also measured (extra keys, not the headline value):
+  * edit_frame: on the same world, frames of 10k sphere edits each followed by
+    one replay of the frame graph (dirty/idle lists, remesh, RLE);
+  * world2k: the 2048x256x2048 world -- batched remesh against the plain
+    per-chunk path (one launch per chunk from the C host loop), the splat
+    resolve of its level-0 list at 1920x1080 over 16 seeded poses, the
+    near-field quad mesh, the octree draw lists, one 64^3 chunk alone;
+  * fit: the 1024x256x1024 colour fit (64 views at 512x512, 200 Adam steps);
+  * roofline of the remesh kernel from HIP events on its launch stream;
+  * cpu_baseline: the CPU oracle (4 OpenMP threads, as chunkset_manage's
+    team) on 512 seeded chunks of the same world.
 
 Prints one JSON line.
 """
@@ -52,22 +55,24 @@ def algorithmic_bytes(uniform, counts) -> int:
     return mixed * (262144 + 3 * 4096) + uni * 3 * 4096 + 12 * records + 20 * len(uniform)
 
 
-def measured_traffic():
-    """HBM bytes per launch measured with rocprofv3 --pmc (committed summary)."""
-    p = ROOT / "profiles" / "cull_mesh_2048x256x2048_traffic.md"
+def measured_traffic(world: str):
+    """HBM bytes per launch of the remesh kernel on `world` (e.g. "16384x512x16384"),
+    measured with rocprofv3 --pmc (committed summary)."""
+    p = ROOT / "profiles" / f"cull_mesh_{world}_traffic.md"
     if not p.exists():
         return None
     m = re.search(r"traffic_bytes_per_launch:\s*([0-9]+)", p.read_text())
     return int(m.group(1)) if m else None
 
 
-def bench_world16k(args) -> int:
-    """The 16384x512x16384 world: `steps` graph launches remeshing all 524,288
-    chunks (the headline value here), then the edit burst and its frame."""
+def leg_world16k(args) -> dict:
+    """The headline: `steps` graph launches remeshing all 524,288 chunks of the
+    16384x512x16384 world, then the edit-burst frames; the CPU baseline on a
+    seeded sample of the same world's chunks."""
     from voxplat_amd import world_frame as WF
 
     def cpu_leg(world):
-        """The oracle on a seeded sample of the world's chunks (their + neighbours
+        """The oracle on 512 seeded chunks of the world (their + neighbours
         copied into a sparse host image), timed on host cores."""
         if args.no_cpu:
             return None
@@ -84,59 +89,42 @@ def bench_world16k(args) -> int:
         smap = world.smap.cpu().numpy().view(np.uint16)
         O.remesh_time(w, hw, sample[:16], smap, args.cpu_threads)       # warm
         t, _ = O.remesh_time(w, hw, sample, smap, args.cpu_threads)
-        return {"value": round(len(sample) * VOXELS_PER_CHUNK / t / 1e9, 4), "unit": "Gvoxels/s",
-                "cores": args.cpu_threads, "kind": "port",
-                "sample": f"512 seeded chunks of the 524,288 (uniform sample, L0..L4 with "
-                          f"shadows), {args.cpu_threads} OpenMP threads as chunkset_manage "
-                          f"(host has {os.cpu_count()} CPUs); {t:.2f} s"}
+        res = {"value": round(len(sample) * VOXELS_PER_CHUNK / t / 1e9, 4), "unit": "Gvoxels/s",
+               "cores": args.cpu_threads, "kind": "port",
+               "sample": f"512 seeded chunks of the 524,288 (uniform sample, L0..L4 with "
+                         f"shadows), {args.cpu_threads} OpenMP threads as chunkset_manage "
+                         f"(host has {os.cpu_count()} CPUs); {t:.2f} s"}
+        log(f"cpu: {res['sample']}: {res['value']} Gvoxel/s")
+        return res
 
-    res = WF.run_world16k(frames=max(1, args.steps // 4), warmup=1, edits=10000, log=log,
-                          full_steps=args.steps, on_generated=cpu_leg)
+    res = WF.run_world16k(frames=args.frames, edits=10000, log=log, full_steps=args.steps,
+                          full_warmup=args.warmup, on_generated=cpu_leg)
     full = res["full_remesh"]
-    line = {
+    gvox = full["gvoxels_s"]
+    return {
         "metric": "Gvoxels/s remeshed (batched vs per-chunk), splats/s at 1080p, % HBM peak",
-        "value": full["gvoxels_s"], "unit": "Gvoxels/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": 1, "ms_per_step": full["ms"], "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "u8",
+        "value": gvox, "unit": "Gvoxels/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": full["ms"], "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "u8",
         "data": "synthetic (seeded value-noise terrain, 5% caves)",
         "config": {"workload": "remesh L0-L4 of 16384x512x16384 (524,288 chunks of 64^3, "
-                               "128 GiB resident), batched persistent kernel in a hipGraph",
+                               "128 GiB resident), batched persistent kernel in a hipGraph, "
+                               "list heaviest first",
                    "chunks": full["chunks"], "occupancy": res["occupancy"],
                    "splat_records": full["records"]},
         "roofline": {"bound": "hbm", "achieved": full["gb_s"], "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(full["gb_s"] / HBM_PEAK_GBS, 4),
-                     "traffic": None, "algorithmic_bytes": full["algorithmic_bytes"],
+                     "traffic": measured_traffic("16384x512x16384"),
+                     "algorithmic_bytes": full["algorithmic_bytes"],
                      "kernel_ms": full["kernel_ms"]},
-        "edit_frame": {k: v for k, v in res.items() if k not in ("full_remesh", "on_generated")},
         "cpu_baseline": res["on_generated"],
+        "edit_frame": {k: v for k, v in res.items()
+                       if k not in ("full_remesh", "on_generated", "occupancy")},
     }
-    print(json.dumps(line))
-    return 0
 
 
-def main() -> int:
-    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--no-splat", action="store_true")
-    ap.add_argument("--no-mesh", action="store_true", help="skip the quad-mesh leg")
-    ap.add_argument("--no-autotune", action="store_true")
-    ap.add_argument("--cpu-threads", type=int, default=4)
-    ap.add_argument("--workload", choices=("world2k", "world16k"), default="world2k",
-                    help="world2k: full remesh of 2048x256x2048 (default); world16k: the "
-                         "16384x512x16384 world, a full remesh plus the edit-burst frame")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported "
-              "(the pipeline runs on a single MI355X; a z-slab split of the chunk grid is how it "
-              "would shard)",
-              file=sys.stderr)
-        return 2
-    if args.workload == "world16k":
-        return bench_world16k(args)
-
+def leg_world2k(args) -> dict:
+    """The 2048x256x2048 legs (see the module docstring)."""
     import numpy as np
     import torch
 
@@ -225,7 +213,7 @@ def main() -> int:
     achieved = algo / (ev_ms * 1e-3) / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                "traffic": measured_traffic(), "algorithmic_bytes": algo,
+                "traffic": measured_traffic("2048x256x2048"), "algorithmic_bytes": algo,
                 "kernel_ms": round(ev_ms, 4)}
 
     # ---- splat resolve at 1080p (level-0 list, chunk order) ----
@@ -449,35 +437,63 @@ def main() -> int:
             log(f"cpu splat: {len(l0h)} splats in {ts:.2f} s")
 
     graph.close()
-    line = {
-        "metric": "Gvoxels/s remeshed (batched vs per-chunk), splats/s at 1080p, % HBM peak",
-        "value": round(gvox, 3),
-        "unit": "Gvoxels/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": round(ms_per_step, 4),
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "u8",
-        "data": "synthetic (seeded value-noise terrain, 5% caves)",
-        "config": {"workload": "remesh L0-L4 of 2048x256x2048 (4096 chunks of 64^3), "
-                               "batched persistent kernel in a hipGraph",
-                   "chunks": nch, "occupancy": occupancy, "splat_records": cap,
-                   "launch": tune},
+    gpu.close()
+    return {
+        "workload": "remesh L0-L4 of 2048x256x2048 (4096 chunks of 64^3), batched persistent "
+                    "kernel in a hipGraph",
+        "chunks": nch, "occupancy": occupancy, "splat_records": cap, "launch": tune,
+        "gvoxels_s": round(gvox, 3), "ms_per_step": round(ms_per_step, 4),
         "per_chunk": {"gvoxels_s": round(pc_gvox, 3), "ms_per_pass": round(pc_ms, 3),
                       "identical_output": same},
         "speedup_vs_per_chunk": round(gvox / pc_gvox, 2),
+        "roofline": roofline,
         "splat": splat_res,
         "mesh": mesh_res,
         "draw": draw_res,
         "cfg0": cfg0_res,
-        "roofline": roofline,
-        "cpu_baseline": cpu,
+        "cpu": cpu,
     }
-    print(json.dumps(line))
+
+
+def leg_fit(args) -> dict:
+    """The 1024x256x1024 colour fit at its stated size."""
+    from voxplat_amd import GPU
+    from voxplat_amd.splat_fit import run_fit
+
+    gpu = GPU(0)
+    res = run_fit(gpu, views=64, size=512, steps=200, lr=0.05, log_every=0)
     gpu.close()
+    log(f"fit: {res['ms_per_step']} ms/step, error {res['mean_abs_rgb_error_visible_init']} -> "
+        f"{res['mean_abs_rgb_error_visible']}")
+    return res
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline legs")
+    ap.add_argument("--no-splat", action="store_true")
+    ap.add_argument("--no-mesh", action="store_true", help="skip the quad-mesh leg")
+    ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--no-2k", action="store_true", help="skip the 2048x256x2048 legs")
+    ap.add_argument("--no-fit", action="store_true", help="skip the colour fit leg")
+    ap.add_argument("--frames", type=int, default=3, help="edit-burst frames on the 16k world")
+    ap.add_argument("--cpu-threads", type=int, default=4)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported "
+              "(the pipeline runs on a single MI355X; a z-slab split of the chunk grid is how it "
+              "would shard)",
+              file=sys.stderr)
+        return 2
+    line = leg_world16k(args)
+    if not args.no_2k:
+        line["world2k"] = leg_world2k(args)
+    if not args.no_fit:
+        line["fit"] = leg_fit(args)
+    print(json.dumps(line))
     return 0
 
 

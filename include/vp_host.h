@@ -56,6 +56,43 @@ int vp_host_remesh_graph_create(vp_ctx *ctx, const vp_world_desc *world,
 int vp_host_remesh_graph_launch(vp_remesh_graph *graph, void *stream);
 int vp_host_remesh_graph_destroy(vp_remesh_graph *graph);
 
+/* One frame of chunk management over a resident world, captured once and
+ * replayed every frame: chunkset_manage's walk over the chunks
+ * (src/chunkset.c:246-313), which remeshes each dirty chunk (clearing its
+ * dirty flag) and compresses every other one (chunk_compress,
+ * chunkset.c:213-230,300-306).  Each replay
+ *   1. turns the dirty bytes d_dirty[chunks] (set by vp_gpu_edit_spheres) into
+ *      d_dirty_ids (the dirty chunks, ascending) and d_idle_ids (every other
+ *      chunk, ascending), with their counts in d_counts[0] and d_counts[1],
+ *      and clears the dirty bytes;
+ *   2. remeshes the dirty list as vp_gpu_cull_mesh does with d_n = d_counts:
+ *      records to remesh.d_splats, offsets and level counts per entry;
+ *   3. RLE-encodes the idle list as vp_gpu_rle does with n = d_counts[1]:
+ *      the look-back that places each stream also sizes it, so one pass
+ *      writes d_word_offsets (entries + 1) and the words that fit
+ *      capacity_words (VP_STATUS_OVERFLOW otherwise), and d_idle_uniform.
+ * Arrays indexed by list entry must hold `chunks` entries (chunks + 1 for
+ * offsets, 5 * chunks for level counts).  The graph owns its scratch (list
+ * counts, tickets and look-back words of both persistent kernels) and frees
+ * it on destroy; replays of one graph run one at a time. */
+typedef struct {
+    uint32_t    *d_dirty_ids;
+    uint32_t    *d_idle_ids;
+    uint32_t    *d_counts;          /* [0] dirty entries, [1] idle entries      */
+    vp_splat_out remesh;
+    uint32_t    *d_words;
+    uint64_t     capacity_words;
+    uint64_t    *d_word_offsets;
+    int16_t     *d_idle_uniform;    /* nullable */
+} vp_frame_out;
+typedef struct vp_frame_graph vp_frame_graph;
+int vp_host_frame_graph_create(vp_ctx *ctx, const vp_world_desc *world, uint8_t *d_voxels,
+                               const int16_t *d_uniform, const vp_shadow_desc *shadow,
+                               uint8_t *d_dirty, uint32_t flags, const vp_frame_out *out,
+                               vp_frame_graph **graph);
+int vp_host_frame_graph_launch(vp_frame_graph *graph, void *stream);
+int vp_host_frame_graph_destroy(vp_frame_graph *graph);
+
 /* Autotuner: time every launch shape on the given workload (reps runs each,
  * median), keep the fastest in the context.  Returns it and its time. */
 int vp_host_autotune_remesh(vp_ctx *ctx, const vp_world_desc *world,
@@ -64,6 +101,34 @@ int vp_host_autotune_remesh(vp_ctx *ctx, const vp_world_desc *world,
                             uint32_t n, uint32_t flags, const vp_splat_out *out, int reps,
                             void *stream, int *best_block, int *best_wg_per_cu,
                             float *best_ms);
+
+/* ---- on-disk RLE world -------------------------------------------------------
+ * The file of command_export (src/deadcode.c:320-346), read back as
+ * chunkset_clear_import does (src/deadcode.c:159-229).  Little endian:
+ *   0x89 "VOXPLAT", root_bitw, max_bitw[3]          (12 bytes)
+ *   every chunk's RLE stream in chunk-index order, u32 words
+ *   (value << 24 | run) with the 0 terminator      (rle.c:44-87)
+ *   optionally the shadow map, size_x * size_z u16  (shadow.h:26-42)
+ * Unlike the dead export, each stream is written whole (its two-byte length
+ * scan stops inside the first 32-bit word) and the map entirely (it writes
+ * shadow_map_length bytes of a u16 map).  The streams are made and expanded
+ * by vp_gpu_rle in batches of chunks; the calls are synchronous on `stream`
+ * and read and clear the context's status word.  Errors: VP_ERR_ARG (bad
+ * argument, file not openable or not writable), VP_ERR_DATA (bad header, a
+ * world other than `world`, a truncated or malformed stream, a trailer that
+ * is neither empty nor a whole map, a map asked for that the file lacks). */
+int vp_host_world_save(vp_ctx *ctx, const char *path, const vp_world_desc *world,
+                       const uint8_t *d_voxels, const uint16_t *d_shadow_map, void *stream,
+                       uint64_t *bytes_written);
+/* The world geometry in a file's header. */
+int vp_host_world_read_header(const char *path, vp_world_desc *world);
+/* Decode every chunk into d_voxels (chunk-major, as vp_gpu_cull_mesh reads
+ * it); d_uniform (nullable) receives -1 or the value of each chunk whose
+ * stream is one full-chunk run; d_shadow_map (nullable) receives the map and
+ * zero padding up to vp_shadow_padded_len entries. */
+int vp_host_world_load(vp_ctx *ctx, const char *path, const vp_world_desc *world,
+                       uint8_t *d_voxels, int16_t *d_uniform, uint16_t *d_shadow_map,
+                       void *stream);
 
 /* ---- frame draw queue ------------------------------------------------------
  * Frustum planes of a view-projection matrix (column-major), normalised, in

@@ -16,6 +16,8 @@
  *                       src/chunkset/shadow.h:77-89)
  *  vp_gpu_edit_spheres  sphere add/remove edits applied in list order
  *                       (src/chunkset/edit.c:179-244); marks touched chunks
+ *  vp_gpu_shadow_spheres the edits' shadow_place_update calls, in their order
+ *                       (src/chunkset/shadow.h:77-89, edit.c:210-236)
  */
 #ifndef VP_WORKLOAD_H
 #define VP_WORKLOAD_H
@@ -60,6 +62,14 @@ typedef struct {
 int vp_gpu_edit_spheres(vp_ctx *ctx, const vp_world_desc *world, uint8_t *d_voxels,
                         const vp_sphere *spheres, uint32_t n, uint8_t *d_dirty,
                         int16_t *d_uniform, void *stream);
+
+/* Device twin of vp_host_shadow_spheres: the same placements in the same
+ * order on the device map, so edits need no host copy of it.  Map rows (fixed
+ * wz) are independent and each is updated by one wave with the row in LDS;
+ * spheres[0..n) is a host array.  Needs size_x = X + Y <= 32768
+ * (VP_ERR_UNSUPPORTED otherwise).  Synchronous on `stream`. */
+int vp_gpu_shadow_spheres(vp_ctx *ctx, const vp_world_desc *world, uint16_t *d_map,
+                          const vp_sphere *spheres, uint32_t n, void *stream);
 
 /* Host: shadow_place_update for every voxel the add-spheres cover, in the
  * order of edit.c:210-236 (chunk list cx, cy, cz; then x, y, z), on a host

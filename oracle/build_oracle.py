@@ -1,8 +1,10 @@
 """Build the CPU oracle (test infrastructure): oracle/libvp_oracle.so.
 
-The reference's own sources for this path cannot be compiled here without
-stand-ins (src/chunkset/mesher.c includes ctx.h -> <GLFW/glfw3.h>; rle.c
-links mem.c, which includes ctx.h too), so there is no oracle/_ref build.
+The reference's own sources for this path cannot be built here without
+stand-ins: src/chunkset/mesher.c calls chunk_open_ro and chunkset_edit_read
+from chunkset.c and edit.c, which include <cglm/cglm.h> (absent); rle.c needs
+mem.c, whose shell commands leave gfx_shell_exclusive_draw (OpenGL) and
+ctx_terminate (GLFW) unresolved.  So there is no oracle/_ref build.
 """
 from __future__ import annotations
 

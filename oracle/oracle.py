@@ -221,6 +221,59 @@ def rle_decompress(words: np.ndarray, cap: int):
     return out, n
 
 
+WORLD_MAGIC = bytes([0x89]) + b"VOXPLAT"
+
+
+def world_file(w: WorldDesc, voxels: np.ndarray, smap: np.ndarray | None = None) -> bytes:
+    """The on-disk RLE world as command_export lays it out (src/deadcode.c:320-346):
+    0x89 "VOXPLAT", root_bitw, max_bitw[3], then every chunk's RLE stream in
+    chunk-index order, then the shadow map.  Each stream is written whole
+    (its u32 words and the 0 terminator, little endian) and the map as all of
+    its size_x * size_z u16 entries, where the dead export's two-byte scan and
+    byte count would cut both short."""
+    R3 = w.R ** 3
+    vox = np.ascontiguousarray(voxels, np.uint8).reshape(-1)
+    parts = [WORLD_MAGIC + bytes([w.root_bitw, *w.max_bitw])]
+    for c in range(w.chunks):
+        parts.append(rle_compress(vox[c * R3:(c + 1) * R3]).astype("<u4").tobytes())
+    if smap is not None:
+        sx, sz = w.shadow_dims()
+        parts.append(np.ascontiguousarray(smap, np.uint16).reshape(-1)[:sx * sz]
+                     .astype("<u2").tobytes())
+    return b"".join(parts)
+
+
+def world_file_parse(data: bytes):
+    """Read a world file back as chunkset_clear_import walks it
+    (src/deadcode.c:159-229): the header, then for each chunk in index order
+    the words up to and including its 0 terminator, then what follows (the
+    shadow map, or nothing).  Returns (world, voxels, map or None)."""
+    if len(data) < 12 or data[:8] != WORLD_MAGIC:
+        raise ValueError("not a voxplat world file")
+    w = WorldDesc.make(data[8], tuple(data[9:12]))
+    R3 = w.R ** 3
+    words = np.frombuffer(data[12:12 + (len(data) - 12) // 4 * 4], dtype="<u4")
+    ends = np.flatnonzero(words == 0)
+    if len(ends) < w.chunks:
+        raise ValueError("truncated stream section")
+    vox = np.zeros(w.chunks * R3, np.uint8)
+    start = 0
+    for c in range(w.chunks):
+        stream = words[start:ends[c] + 1]
+        out, n = rle_decompress(stream, R3)
+        if n != R3:
+            raise ValueError(f"chunk {c}: stream expands to {n} voxels")
+        vox[c * R3:(c + 1) * R3] = out
+        start = ends[c] + 1
+    rest = data[12 + 4 * start:]
+    sx, sz = w.shadow_dims()
+    if len(rest) == 0:
+        return w, vox, None
+    if len(rest) != 2 * sx * sz:
+        raise ValueError("trailer is neither empty nor a whole shadow map")
+    return w, vox, np.frombuffer(rest, dtype="<u2").astype(np.uint16)
+
+
 def edit_sphere(w: WorldDesc, voxels: np.ndarray, smap: np.ndarray | None, centre, radius: int,
                 voxel: int, dirty: np.ndarray | None = None) -> None:
     sx, _ = w.shadow_dims()

@@ -63,7 +63,7 @@ def cfg4(gpu):
 
     cfg = W.CFG4
     w, vox, splats = F.build_scene(gpu, cfg)
-    cams = F.views(w, vox, 64, 512, cfg.seed + 1)
+    cams = F.views_of(w, vox, 64, 512, cfg.seed + 1)
     n = splats.shape[0]
     g = torch.Generator(device="cuda").manual_seed(cfg.seed)
     gt = torch.rand((n, 3), generator=g, device="cuda")

@@ -1,5 +1,5 @@
-"""The chunk lifecycle on the 2048x256x2048 world: edit, one frame (graph
-remesh of the dirty chunks, RLE encode of the idle ones), then every idle
+"""The chunk lifecycle on the 2048x256x2048 world: edit, one frame (the frame
+graph: remesh of the dirty chunks, RLE encode of the idle ones), then every idle
 chunk dropped and opened again from its stream (the reference's
 chunk_compress / chunk_open_*, src/chunkset.c:133-230,300-306).
 
@@ -24,16 +24,21 @@ def test_drop_and_reopen_idle_chunks(gpu):
     print("[gpu-case] lifecycle: 2048x256x2048, edits, frame, drop + reopen idle", flush=True)
     world = WF.ResidentWorld(gpu, W.CFG1)
     world.generate()
+    ids = torch.arange(world.w.chunks, dtype=torch.int32, device="cuda")
+    _, offs, _ = gpu.cull_mesh(world.w, world.voxels, ids, shadow_map=world.smap,
+                               uniform=world.uniform, count_only=True)
+    _, woffs, _ = gpu.rle_encode(world.w, world.voxels, ids, count_only=True)
+    torch.cuda.synchronize()
+    fr = WF.Frame(world, int(offs[-1].item()) // 4, 2 * int(woffs[-1].item()))
     spheres = W.edit_burst(world.w, n=200, seed=7)
     world.edit(spheres)
-    dirty, idle = world.lists()
-    assert dirty.numel() > 0 and idle.numel() > 0
-    fr = WF.Frame(world, dirty, idle)
     fr.run()
-    torch.cuda.synchronize()
+    fr.sync()
+    dirty, idle = fr.dirty.clone(), fr.idle.clone()
+    assert dirty.numel() > 0 and idle.numel() > 0
     assert gpu.status() == 0
     vox0, uni0 = world.voxels.clone(), world.uniform.clone()
-    recs0, words0 = fr.splats.clone(), fr.words.clone()
+    recs0, words0 = fr.splats[:fr.records].clone(), fr.words[:fr.words_n].clone()
     # encode's per-entry uniform values are the world's metadata of those chunks
     assert torch.equal(fr.idle_uniform, world.uniform[idle.long()])
 
@@ -64,12 +69,15 @@ def test_drop_and_reopen_idle_chunks(gpu):
     n = int(o0[-1].item()) // 4
     assert torch.equal(r1[:n], r0[:n])
 
-    # a second frame over the reopened world reproduces the first
+    # a second frame over the reopened world, the same chunks marked dirty,
+    # reproduces the first
+    world.dirty[dirty.long()] = 1
     fr.run()
-    torch.cuda.synchronize()
+    fr.sync()
     assert gpu.status() == 0
-    assert torch.equal(fr.splats, recs0)
-    assert torch.equal(fr.words, words0)
+    assert torch.equal(fr.dirty, dirty) and torch.equal(fr.idle, idle)
+    assert torch.equal(fr.splats[:fr.records], recs0)
+    assert torch.equal(fr.words[:fr.words_n], words0)
 
     # a few reopened chunks against the oracle's RLE of the same voxels
     for k in range(0, int(idle.numel()), max(1, int(idle.numel()) // 6)):

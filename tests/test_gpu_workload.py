@@ -81,3 +81,41 @@ def test_edit_spheres_match_sequential_edits(gpu):
     assert np.array_equal(host(dirty), exp_dirty)
     assert np.array_equal(host(uni)[exp_dirty == 1], np.full(exp_dirty.sum(), -1))
     assert np.array_equal(m_gpu, smap_h)
+
+
+@pytest.mark.parametrize("rb,mb,n,rmax,seed", [(4, (2, 2, 2), 60, 9, 5), (3, (3, 1, 3), 200, 12, 6),
+                                               (6, (2, 1, 2), 300, 32, 7)])
+def test_device_shadow_spheres_match_sequential_order(gpu, rb, mb, n, rmax, seed):
+    """vp_gpu_shadow_spheres against the oracle's sequential edits (chunk
+    list cx, cy, cz; then x, y, z): the whole map, bit-exact.  Spheres reach
+    past the world on every side, and overlap."""
+    cfg = W.small_world(rb, mb, 31 + seed)
+    w = cfg.desc()
+    vox_h = O.gen_world(w, cfg.gen())
+    smap_h = O.shadow_build(w, vox_h)
+    spheres = W.edit_burst(w, n=n, seed=seed, rmin=1, rmax=rmax)
+    md = dev(smap_h)
+    gpu.shadow_spheres(w, md, spheres)
+    for s in spheres:
+        O.edit_sphere(w, vox_h, smap_h, (s.cx, s.cy, s.cz), s.radius, s.voxel)
+    assert np.array_equal(host(md, np.uint16), smap_h)
+    assert gpu.status() == 0
+
+
+def test_device_shadow_spheres_cfg1_burst(gpu):
+    """2000 spheres (r 1-32) on the 2048x256x2048 world: device map equals the
+    host C placement in the same order."""
+    import torch
+
+    cfg = W.CFG1
+    w = cfg.desc()
+    vd = torch.empty(w.chunks * w.R ** 3, dtype=torch.uint8, device="cuda")
+    gpu.generate(w, cfg.gen(), vd)
+    sx, sz = w.shadow_dims()
+    md = torch.empty(abi.shadow_padded_len(sx, sz), dtype=torch.int16, device="cuda")
+    gpu.shadow_build(w, vd, md)
+    spheres = W.edit_burst(w, n=2000, seed=11)
+    m_host = host(md, np.uint16).copy()
+    gpu.shadow_spheres(w, md, spheres)
+    host_shadow_spheres(w, m_host, spheres)
+    assert np.array_equal(host(md, np.uint16), m_host)

@@ -1,16 +1,19 @@
 """The 16384x512x16384 resident world (524,288 chunks, 128 GiB in HBM):
-generate, 10k sphere edits, one frame of batched graph remesh of the dirty
-chunks plus RLE encode of every other chunk.
+generate, then three frames, each a burst of 10k sphere edits (voxels and
+shadow map on the device) followed by one replay of the frame graph captured
+once: dirty/idle lists on the device, batched remesh of the dirty chunks, RLE
+encode of every other chunk.
 
-Checked at full size:
-  * the dirty set equals every chunk of every sphere's box (edit.c:98-112,241);
+Checked at full size, on the last frame:
+  * the graph's dirty list equals every chunk of every box of the last
+    burst's spheres (edit.c:98-112,241), and the dirty bytes are cleared;
   * every idle chunk's RLE stream decodes on the GPU back to its voxels;
-  * the count pass and the graph's emission agree on every offset;
+  * a direct count pass over the dirty list agrees with every offset;
 and against the oracle on sampled chunks (the world does not fit the host):
   * remesh records and level counts of 32 dirty chunks, bit-exact;
   * RLE streams of 32 idle chunks, bit-exact;
   * the voxels of 12 dirty chunks, regenerated on the CPU and edited by the
-    spheres that reach them in order, bit-exact."""
+    spheres of all bursts that reach them, in order, bit-exact."""
 import numpy as np
 import pytest
 
@@ -41,29 +44,34 @@ def world16k(gpu):
     import torch
     from voxplat_amd import world_frame as WF
 
-    print("[gpu-case] world16k: generate 16384x512x16384, 10k edits, one frame", flush=True)
-    res = WF.run_world16k(frames=1, warmup=0, edits=10000, keep=True, gpu=gpu)
-    world, fr, spheres = res.pop("_objects")
+    print("[gpu-case] world16k: generate 16384x512x16384, 3 frames of 10k edits", flush=True)
+    res = WF.run_world16k(frames=3, edits=10000, keep=True, gpu=gpu)
+    world, fr, every, last = res.pop("_objects")
     print(f"[gpu-case] world16k: {res}", flush=True)
-    yield res, world, fr, spheres
+    yield res, world, fr, (every, last)
     fr.close()
     del world, fr
     torch.cuda.empty_cache()
 
 
 def test_frame_status_and_dirty_set(world16k):
-    res, world, fr, spheres = world16k
+    res, world, fr, (every, last) = world16k
     assert res["status"] == 0
-    ed = _expected_dirty(world.w, spheres)
-    assert np.array_equal(host(world.dirty).astype(bool), ed)
-    assert res["dirty_chunks"] == int(ed.sum())
-    assert res["dirty_chunks"] + res["idle_chunks"] == world.w.chunks
+    assert all(f["status"] == 0 for f in res["frames"])
+    ed = _expected_dirty(world.w, last)
+    assert np.array_equal(host(fr.dirty), np.flatnonzero(ed).astype(np.int32))
+    assert not host(world.dirty).any()
+    f = res["frames"][-1]
+    assert f["dirty_chunks"] == int(ed.sum())
+    assert f["dirty_chunks"] + f["idle_chunks"] == world.w.chunks
+    # the bursts differ, and so do their dirty sets
+    assert len({p["dirty_chunks"] for p in res["frames"]}) > 1
 
 
 def test_remesh_offsets_match_count_pass(gpu, world16k):
     import torch
 
-    res, world, fr, spheres = world16k
+    res, world, fr, (every, last) = world16k
     # the graph's emission rewrote offsets/counts; a fresh count pass must agree
     _, offs, counts = gpu.cull_mesh(world.w, world.voxels, fr.dirty, shadow_map=world.smap,
                                     uniform=world.uniform, count_only=True)
@@ -75,7 +83,7 @@ def test_remesh_offsets_match_count_pass(gpu, world16k):
 def test_remesh_sampled_chunks_match_oracle(world16k):
     from voxplat_amd import world_frame as WF
 
-    res, world, fr, spheres = world16k
+    res, world, fr, (every, last) = world16k
     w = world.w
     dirty_h = host(fr.dirty).astype(np.uint32)
     rng = np.random.default_rng(0x16C)
@@ -93,7 +101,7 @@ def test_remesh_sampled_chunks_match_oracle(world16k):
 
 
 def test_rle_sampled_chunks_match_oracle(world16k):
-    res, world, fr, spheres = world16k
+    res, world, fr, (every, last) = world16k
     idle_h = host(fr.idle).astype(np.uint32)
     rng = np.random.default_rng(0x16D)
     uni = host(fr.idle_uniform)
@@ -111,7 +119,7 @@ def test_rle_every_idle_chunk_decodes_to_its_voxels(gpu, world16k):
     world and compared with the resident voxels."""
     import torch
 
-    res, world, fr, spheres = world16k
+    res, world, fr, (every, last) = world16k
     R3 = world.R3
     m = 4096
     scratch_w = abi.WorldDesc.make(6, (5, 2, 5))
@@ -131,7 +139,7 @@ def test_rle_every_idle_chunk_decodes_to_its_voxels(gpu, world16k):
 def test_edits_sampled_chunks_match_oracle(world16k):
     from voxplat_amd import world_frame as WF
 
-    res, world, fr, spheres = world16k
+    res, world, fr, (every, last) = world16k
     w = world.w
     b = w.root_bitw
     dirty_h = host(fr.dirty).astype(np.uint32)
@@ -144,7 +152,7 @@ def test_edits_sampled_chunks_match_oracle(world16k):
         cc = (c & ((1 << mxb[0]) - 1), (c >> mxb[0]) & ((1 << mxb[1]) - 1), c >> (mxb[0] + mxb[1]))
         R3 = world.R3
         hw[c * R3:(c + 1) * R3] = O.gen_chunk(w, params, c)
-        for s in spheres:
+        for s in every:
             ctr = (s.cx, s.cy, s.cz)
             if all(((ctr[i] - s.radius - 1) >> b) <= cc[i] <= ((ctr[i] + s.radius + 1) >> b)
                    for i in range(3)):

@@ -5,8 +5,8 @@ include/vp_gpu.h.
     from voxplat_amd import GPU, abi, workload
 """
 from . import abi, workload
-from ._lib import (GPU, RemeshGraph, VPError, frustum_planes, host_shadow_spheres, load_library,
-                   octree_cull)
+from ._lib import (GPU, FrameGraph, RemeshGraph, VPError, frustum_planes, host_shadow_spheres, load_library,
+                   octree_cull, world_read_header)
 
-__all__ = ["GPU", "RemeshGraph", "VPError", "abi", "workload", "load_library", "host_shadow_spheres",
-           "frustum_planes", "octree_cull"]
+__all__ = ["GPU", "FrameGraph", "RemeshGraph", "VPError", "abi", "workload", "load_library", "host_shadow_spheres",
+           "frustum_planes", "octree_cull", "world_read_header"]

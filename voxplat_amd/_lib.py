@@ -493,6 +493,15 @@ class GPU:
                                             C.c_void_p(self._stream(stream))),
                "vp_gpu_edit_spheres")
 
+    def shadow_spheres(self, world: abi.WorldDesc, smap, spheres, stream=None):
+        """The add-spheres' shadow_place_update calls, in order, on the device map."""
+        sx, sz = world.shadow_dims()
+        _need(smap, abi.shadow_padded_len(sx, sz) * 2, "shadow_map")
+        arr = (abi.Sphere * len(spheres))(*spheres)
+        _check(self.lib.vp_gpu_shadow_spheres(self.ctx, C.byref(world), C.c_void_p(smap.data_ptr()),
+                                              arr, len(spheres), C.c_void_p(self._stream(stream))),
+               "vp_gpu_shadow_spheres")
+
 
 def _draw_dtype():
     import numpy as np
@@ -586,6 +595,73 @@ class RemeshGraph:
             pass
 
 
+class FrameGraph:
+    """One frame of chunk management over a resident world, captured once
+    (vp_host_frame_graph_*, vp_host.h): the dirty bytes become a dirty and an
+    idle list on the device, the dirty list is remeshed and the idle list
+    RLE-encoded.  Output buffers are allocated here for `record_capacity`
+    records and `word_capacity` RLE words; the lists and counts are
+    re-made by every launch."""
+
+    def __init__(self, gpu: GPU, world: abi.WorldDesc, voxels, uniform, shadow_map, dirty,
+                 record_capacity: int, word_capacity: int, lod: bool = True):
+        torch = gpu.torch
+        self.gpu, self.world = gpu, world
+        nch = world.chunks
+        dev = voxels.device
+        _need_world(world, voxels, uniform, shadow_map)
+        _need(dirty, nch, "dirty")
+        self.dirty_ids = torch.empty(nch, dtype=torch.int32, device=dev)
+        self.idle_ids = torch.empty(nch, dtype=torch.int32, device=dev)
+        self.counts = torch.zeros(2, dtype=torch.int32, device=dev)
+        self.offsets = torch.zeros(nch + 1, dtype=torch.int64, device=dev)
+        self.level_counts = torch.zeros((nch, 5), dtype=torch.int32, device=dev)
+        self.splats = torch.empty((max(record_capacity, 1), 4), dtype=torch.int16, device=dev)
+        self.words = torch.empty(max(word_capacity, 1), dtype=torch.int32, device=dev)
+        self.word_offsets = torch.zeros(nch + 1, dtype=torch.int64, device=dev)
+        self.idle_uniform = torch.empty(nch, dtype=torch.int16, device=dev)
+        self.record_capacity, self.word_capacity = record_capacity, word_capacity
+        sh = abi.ShadowDesc()
+        if shadow_map is not None:
+            sx, sz = world.shadow_dims()
+            sh.d_map, sh.size_x, sh.size_z = shadow_map.data_ptr(), sx, sz
+        fo = abi.FrameOut()
+        fo.d_dirty_ids, fo.d_idle_ids = self.dirty_ids.data_ptr(), self.idle_ids.data_ptr()
+        fo.d_counts = self.counts.data_ptr()
+        fo.remesh.d_splats, fo.remesh.capacity_records = self.splats.data_ptr(), record_capacity
+        fo.remesh.d_chunk_offsets = self.offsets.data_ptr()
+        fo.remesh.d_level_counts = self.level_counts.data_ptr()
+        fo.d_words, fo.capacity_words = self.words.data_ptr(), word_capacity
+        fo.d_word_offsets, fo.d_idle_uniform = self.word_offsets.data_ptr(), self.idle_uniform.data_ptr()
+        self._keep = (world, voxels, uniform, shadow_map, dirty, sh, fo)
+        h = C.c_void_p()
+        _check(gpu.lib.vp_host_frame_graph_create(
+            gpu.ctx, C.byref(world), C.c_void_p(voxels.data_ptr()), C.c_void_p(_ptr(uniform)),
+            C.byref(sh), C.c_void_p(dirty.data_ptr()), abi.VP_CM_LOD if lod else 0, C.byref(fo),
+            C.byref(h)), "vp_host_frame_graph_create")
+        self.h = h
+
+    def launch(self, stream=None):
+        _check(self.gpu.lib.vp_host_frame_graph_launch(self.h, C.c_void_p(self.gpu._stream(stream))),
+               "vp_host_frame_graph_launch")
+
+    def lists(self):
+        """(dirty ids, idle ids) of the last replay, as device views."""
+        nd, ni = (int(v) for v in self.counts.cpu().tolist())
+        return self.dirty_ids[:nd], self.idle_ids[:ni]
+
+    def close(self):
+        if self.h:
+            self.gpu.lib.vp_host_frame_graph_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def _gpu_reserve(self: GPU, max_chunks: int, max_pixels: int = 0) -> None:
     _check(self.lib.vp_ctx_reserve(self.ctx, max_chunks, max_pixels), "vp_ctx_reserve")
 
@@ -639,6 +715,38 @@ def _gpu_autotune(self: GPU, world: abi.WorldDesc, voxels, ids, shadow_map, spla
     return b.value, w.value, ms.value
 
 
+def _gpu_world_save(self: GPU, path, world: abi.WorldDesc, voxels, shadow_map=None,
+                    stream=None) -> int:
+    """Write the on-disk RLE world (vp_host_world_save); returns the bytes written."""
+    _need_world(world, voxels, None, shadow_map)
+    n = C.c_uint64()
+    _check(self.lib.vp_host_world_save(
+        self.ctx, str(path).encode(), C.byref(world), C.c_void_p(voxels.data_ptr()),
+        C.c_void_p(_ptr(shadow_map)), C.c_void_p(self._stream(stream)), C.byref(n)),
+        "vp_host_world_save")
+    return n.value
+
+
+def world_read_header(path) -> abi.WorldDesc:
+    """The world geometry in an on-disk RLE world's header."""
+    w = abi.WorldDesc()
+    _check(load_library().vp_host_world_read_header(str(path).encode(), C.byref(w)),
+           "vp_host_world_read_header")
+    return w
+
+
+def _gpu_world_load(self: GPU, path, world: abi.WorldDesc, voxels, uniform=None, shadow_map=None,
+                    stream=None) -> None:
+    """Decode an on-disk RLE world into `voxels` (and `uniform`, `shadow_map`)."""
+    _need_world(world, voxels, uniform, shadow_map)
+    _check(self.lib.vp_host_world_load(
+        self.ctx, str(path).encode(), C.byref(world), C.c_void_p(voxels.data_ptr()),
+        C.c_void_p(_ptr(uniform)), C.c_void_p(_ptr(shadow_map)),
+        C.c_void_p(self._stream(stream))), "vp_host_world_load")
+
+
+GPU.world_save = _gpu_world_save
+GPU.world_load = _gpu_world_load
 GPU.reserve = _gpu_reserve
 GPU.set_launch = _gpu_set_launch
 GPU.get_launch = _gpu_get_launch

@@ -170,11 +170,25 @@ PROTOTYPES.update({
                                       C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p]),
     "vp_host_shadow_spheres": (C.c_int, [C.POINTER(WorldDesc), C.c_void_p, C.c_void_p,
                                          C.c_uint32]),
+    "vp_gpu_shadow_spheres": (C.c_int, [C.c_void_p, C.POINTER(WorldDesc), C.c_void_p, C.c_void_p,
+                                        C.c_uint32, C.c_void_p]),
 })
 
 
 # ---- include/vp_host.h ---------------------------------------------------------
+class FrameOut(C.Structure):
+    _fields_ = [("d_dirty_ids", C.c_void_p), ("d_idle_ids", C.c_void_p), ("d_counts", C.c_void_p),
+                ("remesh", SplatOut), ("d_words", C.c_void_p), ("capacity_words", C.c_uint64),
+                ("d_word_offsets", C.c_void_p), ("d_idle_uniform", C.c_void_p)]
+
+
 PROTOTYPES.update({
+    "vp_host_frame_graph_create": (C.c_int, [C.c_void_p, C.POINTER(WorldDesc), C.c_void_p,
+                                             C.c_void_p, C.POINTER(ShadowDesc), C.c_void_p,
+                                             C.c_uint32, C.POINTER(FrameOut),
+                                             C.POINTER(C.c_void_p)]),
+    "vp_host_frame_graph_launch": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "vp_host_frame_graph_destroy": (C.c_int, [C.c_void_p]),
     "vp_ctx_reserve": (C.c_int, [C.c_void_p, C.c_uint64, C.c_uint64]),
     "vp_ctx_set_launch": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
     "vp_ctx_get_launch": (C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
@@ -187,6 +201,11 @@ PROTOTYPES.update({
                                               C.c_uint32, C.c_void_p, C.c_uint32,
                                               C.POINTER(SplatOut), C.POINTER(C.c_void_p)]),
     "vp_host_remesh_graph_launch": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "vp_host_world_save": (C.c_int, [C.c_void_p, C.c_char_p, C.POINTER(WorldDesc), C.c_void_p,
+                                     C.c_void_p, C.c_void_p, C.POINTER(C.c_uint64)]),
+    "vp_host_world_read_header": (C.c_int, [C.c_char_p, C.POINTER(WorldDesc)]),
+    "vp_host_world_load": (C.c_int, [C.c_void_p, C.c_char_p, C.POINTER(WorldDesc), C.c_void_p,
+                                     C.c_void_p, C.c_void_p, C.c_void_p]),
     "vp_host_remesh_graph_destroy": (C.c_int, [C.c_void_p]),
     "vp_host_autotune_remesh": (C.c_int, [C.c_void_p, C.POINTER(WorldDesc), C.c_void_p,
                                           C.c_void_p, C.POINTER(ShadowDesc), C.c_void_p,

@@ -259,6 +259,67 @@ __global__ void k_mark(const PairList *lists, uint32_t n, uint8_t *dirty, int16_
     }
 }
 
+// shadow_place_update for add-sphere voxels (src/chunkset/shadow.h:77-89,
+// called from chunkset_edit_sphere, src/chunkset/edit.c:210-236), on the
+// device.  A placement at (wx, wy, wz) reads entries i = wx + wy + sx * wz
+// and i + 1 and may write entry i, all in map row wz, so rows are
+// independent: one wave per row, the row staged in LDS.  In a row the
+// reference's order is: spheres in list order; chunks cx, then cy (cz is the
+// row's); then ux; then uy.  One "sweep" (fixed sphere, chunk, ux) places
+// uy = y0.. in order at i = ux + uy, and each placement reads i and i + 1,
+// which no earlier placement of the same sweep writes (they write below i):
+// so the lanes of a wave take the uy of a sweep together, all reads of the
+// sweep before its writes, and the sweeps follow each other in order.
+__global__ void __launch_bounds__(64) k_shadow_spheres(uint16_t *map, uint32_t sx, int b,
+                                                       int32_t mx0, int32_t mx1,
+                                                       const uint32_t *row_off,
+                                                       const uint32_t *row_sph,
+                                                       const vp_sphere *sph)
+{
+    extern __shared__ uint16_t row[];
+    const uint32_t wz = blockIdx.x, lane = threadIdx.x;
+    const uint32_t s0 = row_off[wz], s1 = row_off[wz + 1];
+    if (s0 == s1) return;
+    const int32_t R = 1 << b;
+    uint32_t *row32 = reinterpret_cast<uint32_t *>(row);
+    const uint32_t *src = reinterpret_cast<const uint32_t *>(map + (size_t)sx * wz);
+    for (uint32_t e = lane; e < sx / 2; e += 64) row32[e] = src[e];
+    __syncthreads();
+    for (uint32_t k = s0; k < s1; k++) {
+        const vp_sphere sp = sph[row_sph[k]];
+        const int32_t c[3] = {sp.cx, sp.cy, sp.cz}, r = (int32_t)sp.radius;
+        const int64_t dz = (int64_t)wz - c[2];
+        const int64_t rz2 = (int64_t)r * r - dz * dz;      // > 0 for a listed sphere
+        const int32_t lo0 = c[0] - r - 1, hi0 = c[0] + r + 1, lo1 = c[1] - r - 1, hi1 = c[1] + r + 1;
+        for (int32_t cx = lo0 >> b; cx <= (hi0 >> b); cx++) {
+            if (cx < 0 || cx >= mx0) continue;
+            const int32_t x0 = max(lo0, cx * R), x1 = min(hi0, cx * R + R);
+            for (int32_t cy = lo1 >> b; cy <= (hi1 >> b); cy++) {
+                if (cy < 0 || cy >= mx1) continue;
+                const int32_t y0 = max(lo1, cy * R), y1 = min(hi1, cy * R + R);
+                for (int32_t ux = x0; ux < x1; ux++) {
+                    const int64_t dx = (int64_t)ux - c[0];
+                    if (dx * dx >= rz2) continue;
+                    const int32_t uy = y0 + (int32_t)lane;
+                    const int64_t dy = (int64_t)uy - c[1];
+                    if (uy < y1 && dx * dx + dy * dy < rz2) {
+                        const uint32_t i = (uint32_t)(ux + uy), wy = (uint32_t)uy;
+                        const bool lit = row[i] <= wy && row[i + 1] <= wy;
+                        __builtin_amdgcn_wave_barrier();    // every read of the sweep first
+                        if (lit) row[i] = (uint16_t)wy;
+                    }
+                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                    __builtin_amdgcn_wave_barrier();
+                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                }
+            }
+        }
+    }
+    __syncthreads();
+    uint32_t *dst = reinterpret_cast<uint32_t *>(map + (size_t)sx * wz);
+    for (uint32_t e = lane; e < sx / 2; e += 64) dst[e] = row32[e];
+}
+
 template <typename F>
 static hipError_t by_root(int b, F f)
 {
@@ -430,6 +491,76 @@ extern "C" int vp_gpu_edit_spheres(vp_ctx *ctx, const vp_world_desc *world, uint
     if (e == hipSuccess) e = hipStreamSynchronize(st);
     (void)hipFree(d_lists);
     (void)hipFree(d_sidx);
+    (void)hipFree(d_sph);
+    return e == hipSuccess ? VP_OK : VP_ERR_HIP;
+}
+
+extern "C" int vp_gpu_shadow_spheres(vp_ctx *ctx, const vp_world_desc *world, uint16_t *d_map,
+                                     const vp_sphere *spheres, uint32_t n, void *stream)
+{
+    if (!ctx || !d_map || (n && !spheres)) return VP_ERR_ARG;
+    int rc = check_world_w(world);
+    if (rc) return rc;
+    const int b = world->root_bitw;
+    const uint32_t X = (1u << world->max_bitw[0]) << b, Y = (1u << world->max_bitw[1]) << b;
+    const uint32_t Z = (1u << world->max_bitw[2]) << b;
+    const uint32_t sx = X + Y;
+    // the row sits in LDS: up to 64 KiB
+    if (sx > 32768u) return VP_ERR_UNSUPPORTED;
+    // per-row sphere lists, in sphere order: the add-spheres whose voxels reach the row
+    std::vector<uint32_t> cnt(Z + 1, 0u);
+    for (uint32_t i = 0; i < n; i++) {
+        const vp_sphere &s = spheres[i];
+        if (!s.voxel || s.radius == 0) continue;
+        const int64_t z0 = std::max<int64_t>((int64_t)s.cz - s.radius + 1, 0);
+        const int64_t z1 = std::min<int64_t>((int64_t)s.cz + s.radius - 1, (int64_t)Z - 1);
+        for (int64_t z = z0; z <= z1; z++) cnt[(size_t)z + 1]++;
+    }
+    for (uint32_t z = 0; z < Z; z++) cnt[z + 1] += cnt[z];
+    if (cnt[Z] == 0) return VP_OK;
+    std::vector<uint32_t> lst(cnt[Z]);
+    {
+        std::vector<uint32_t> fill(cnt.begin(), cnt.end() - 1);
+        for (uint32_t i = 0; i < n; i++) {
+            const vp_sphere &s = spheres[i];
+            if (!s.voxel || s.radius == 0) continue;
+            const int64_t z0 = std::max<int64_t>((int64_t)s.cz - s.radius + 1, 0);
+            const int64_t z1 = std::min<int64_t>((int64_t)s.cz + s.radius - 1, (int64_t)Z - 1);
+            for (int64_t z = z0; z <= z1; z++) lst[fill[(size_t)z]++] = i;
+        }
+    }
+    hipStream_t st = as_stream(stream);
+    uint32_t *d_off = nullptr, *d_lst = nullptr;
+    vp_sphere *d_sph = nullptr;
+    if (hipMalloc(&d_off, cnt.size() * sizeof(uint32_t)) != hipSuccess ||
+        hipMalloc(&d_lst, lst.size() * sizeof(uint32_t)) != hipSuccess ||
+        hipMalloc(&d_sph, n * sizeof(vp_sphere)) != hipSuccess) {
+        (void)hipFree(d_off);
+        (void)hipFree(d_lst);
+        (void)hipFree(d_sph);
+        return VP_ERR_NOMEM;
+    }
+    hipError_t e = hipMemcpyAsync(d_off, cnt.data(), cnt.size() * sizeof(uint32_t),
+                                  hipMemcpyHostToDevice, st);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(d_lst, lst.data(), lst.size() * sizeof(uint32_t), hipMemcpyHostToDevice,
+                           st);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(d_sph, spheres, n * sizeof(vp_sphere), hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) {
+        const size_t lds = (size_t)sx * sizeof(uint16_t);
+        if (lds > 65536) e = hipErrorInvalidValue;
+        else {
+            hipLaunchKernelGGL(k_shadow_spheres, dim3(Z), dim3(64), lds, st, d_map, sx, b,
+                               (int32_t)(1u << world->max_bitw[0]),
+                               (int32_t)(1u << world->max_bitw[1]), d_off, d_lst, d_sph);
+            e = hipGetLastError();
+        }
+    }
+    // the temporary lists are freed below; wait for the kernel that reads them
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    (void)hipFree(d_off);
+    (void)hipFree(d_lst);
     (void)hipFree(d_sph);
     return e == hipSuccess ? VP_OK : VP_ERR_HIP;
 }

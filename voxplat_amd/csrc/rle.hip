@@ -31,6 +31,7 @@ struct RleArgs {
     uint8_t        *voxels;
     const uint32_t *ids;
     uint32_t        n;
+    const uint32_t *d_n;           // nullable: device-resident count <= n (encode)
     uint32_t       *words;
     uint64_t        cap;
     uint64_t       *offsets;
@@ -287,12 +288,13 @@ __global__ void __launch_bounds__(BS) k_rle_encode(RleArgs a)
         rle_encode_chunk<B, BS>(a, 0u, 1u, bm, woff, wnext, wsum, &s_base);
         return;
     }
+    const uint32_t n = a.d_n ? min(*a.d_n, a.n) : a.n;
     for (;;) {
         if (threadIdx.x == 0) s_ticket = atomicAdd(a.ticket, 1u);
         __syncthreads();
         const uint32_t k = s_ticket;
-        if (k >= a.n) break;
-        rle_encode_chunk<B, BS>(a, k, a.n, bm, woff, wnext, wsum, &s_base);
+        if (k >= n) break;
+        rle_encode_chunk<B, BS>(a, k, n, bm, woff, wnext, wsum, &s_base);
         __syncthreads();
     }
 }
@@ -439,6 +441,60 @@ static hipError_t rle_launch(int dir, const RleArgs &a, int grid, hipStream_t st
     return hipGetLastError();
 }
 
+static int rle_check(const vp_world_desc *world)
+{
+    if (!world) return VP_ERR_ARG;
+    if (world->root_bitw < 2 || world->root_bitw > 6) return VP_ERR_UNSUPPORTED;
+    if ((int)world->max_bitw[0] + world->max_bitw[1] + world->max_bitw[2] > 32) return VP_ERR_ARG;
+    return VP_OK;
+}
+
+static hipError_t rle_dispatch(uint32_t root_bitw, int dir, const RleArgs &a, int grid,
+                               hipStream_t st)
+{
+    switch (root_bitw) {
+    case 2: return rle_launch<2>(dir, a, grid, st);
+    case 3: return rle_launch<3>(dir, a, grid, st);
+    case 4: return rle_launch<4>(dir, a, grid, st);
+    case 5: return rle_launch<5>(dir, a, grid, st);
+    default: return rle_launch<6>(dir, a, grid, st);
+    }
+}
+
+int rle_encode_launch(vp_ctx *ctx, const vp_world_desc *world, const uint8_t *d_voxels,
+                      const uint32_t *d_chunk_ids, uint32_t n, const uint32_t *d_n,
+                      uint32_t *d_words, uint64_t capacity_words, uint64_t *d_word_offsets,
+                      int16_t *d_uniform_out, uint32_t *ticket, uint64_t *state, hipStream_t st)
+{
+    int rc = rle_check(world);
+    if (rc) return rc;
+    if (!d_voxels || !d_chunk_ids || !d_word_offsets || !ticket || !state) return VP_ERR_ARG;
+    if (!d_words && capacity_words != 0) return VP_ERR_ARG;
+    if (n == 0) return VP_OK;
+    RleArgs a = {};
+    a.voxels = const_cast<uint8_t *>(d_voxels);
+    a.ids = d_chunk_ids;
+    a.n = n;
+    a.d_n = d_n;
+    a.words = d_words;
+    a.cap = capacity_words;
+    a.offsets = d_word_offsets;
+    a.uniform_out = d_uniform_out;
+    a.status = ctx->d_status;
+    a.single = 0;
+    a.nchunks = (uint32_t)(1ull << (world->max_bitw[0] + world->max_bitw[1] + world->max_bitw[2]));
+    a.ticket = ticket;
+    a.state = state;
+    const int resident = ctx->num_cus * 2;
+    const int grid = (int)(n < (uint32_t)resident ? n : (uint32_t)resident);
+    const int rgrid = (int)((n + 255u) / 256u);
+    hipLaunchKernelGGL(k_rle_reset, dim3(rgrid < 1024 ? rgrid : 1024), dim3(256), 0, st, a.ticket,
+                       a.state, n, a.offsets);
+    VP_HIP_TRY(hipGetLastError());
+    VP_HIP_TRY(rle_dispatch(world->root_bitw, VP_RLE_ENCODE, a, grid, st));
+    return VP_OK;
+}
+
 }  // namespace vp
 
 using namespace vp;
@@ -487,13 +543,5 @@ extern "C" int vp_gpu_rle(vp_ctx *ctx, int dir, const vp_world_desc *world,
         const int resident = ctx->num_cus * 2;
         grid = (int)(n < (uint32_t)resident ? n : (uint32_t)resident);
     }
-    hipError_t e;
-    switch (root_bitw) {
-    case 2: e = rle_launch<2>(dir, a, grid, st); break;
-    case 3: e = rle_launch<3>(dir, a, grid, st); break;
-    case 4: e = rle_launch<4>(dir, a, grid, st); break;
-    case 5: e = rle_launch<5>(dir, a, grid, st); break;
-    default: e = rle_launch<6>(dir, a, grid, st); break;
-    }
-    return e == hipSuccess ? VP_OK : VP_ERR_HIP;
+    return rle_dispatch(root_bitw, dir, a, grid, st) == hipSuccess ? VP_OK : VP_ERR_HIP;
 }

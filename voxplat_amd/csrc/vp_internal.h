@@ -63,6 +63,13 @@ int cull_mesh_launch(vp_ctx *ctx, const vp_world_desc *world, const uint8_t *d_v
                      uint32_t flags, const vp_splat_out *out, uint32_t *ticket,
                      uint64_t *state, hipStream_t st);
 
+// The RLE encode of vp_gpu_rle with explicit scratch (a ticket word and n
+// look-back words) and an optional device-resident count d_n <= n.
+int rle_encode_launch(vp_ctx *ctx, const vp_world_desc *world, const uint8_t *d_voxels,
+                      const uint32_t *d_chunk_ids, uint32_t n, const uint32_t *d_n,
+                      uint32_t *d_words, uint64_t capacity_words, uint64_t *d_word_offsets,
+                      int16_t *d_uniform_out, uint32_t *ticket, uint64_t *state, hipStream_t st);
+
 // relaxed device-scope atomics on 64-bit words (coherent across XCDs)
 __device__ __forceinline__ uint64_t ld_relaxed(const uint64_t *p)
 {

@@ -75,7 +75,7 @@ def build_scene(gpu: GPU, cfg: W.WorldConfig = W.CFG4):
     return w, vox, recs[:n].contiguous()
 
 
-def views(w: abi.WorldDesc, vox, n: int, size: int, seed: int):
+def views_of(w: abi.WorldDesc, vox, n: int, size: int, seed: int):
     X, Y, Z = w.size
     mx, my, mz = (1 << w.max_bitw[0]), (1 << w.max_bitw[1]), (1 << w.max_bitw[2])
     R = w.R
@@ -122,21 +122,15 @@ def visible(gpu: GPU, splats, cams, bounds=None):
     return seen
 
 
-def main(argv=None) -> int:
-    import argparse
-
-    ap = argparse.ArgumentParser(description="per-splat colour fit to rendered target views")
-    ap.add_argument("--views", type=int, default=64)
-    ap.add_argument("--size", type=int, default=512)
-    ap.add_argument("--steps", type=int, default=200)
-    ap.add_argument("--lr", type=float, default=0.05)
-    a = ap.parse_args(argv)
+def run_fit(gpu: GPU, views: int = 64, size: int = 512, steps: int = 200, lr: float = 0.05,
+            log_every: int = 20) -> dict:
+    """The 1024x256x1024 colour fit: seeded ground truth, uniform random
+    initial colours, Adam over `views` target views.  Returns its figures."""
     cfg = W.CFG4
-    gpu = GPU(0)
     t0 = time.perf_counter()
     w, vox, splats = build_scene(gpu, cfg)
     n = splats.shape[0]
-    cams = views(w, vox, a.views, a.size, cfg.seed + 1)
+    cams = views_of(w, vox, views, size, cfg.seed + 1)
     g = torch.Generator(device="cuda").manual_seed(cfg.seed)
     gt = torch.rand((n, 3), generator=g, device="cuda")
     init = torch.rand((n, 3), generator=g, device="cuda")
@@ -147,20 +141,35 @@ def main(argv=None) -> int:
     torch.cuda.synchronize()
     t_setup = time.perf_counter() - t0
     t0 = time.perf_counter()
-    rgb, hist = fit_colours(gpu, splats, cams, targets, init, steps=a.steps, lr=a.lr,
-                            log_every=20, bounds=bounds)
+    rgb, hist = fit_colours(gpu, splats, cams, targets, init, steps=steps, lr=lr,
+                            log_every=log_every, bounds=bounds)
     torch.cuda.synchronize()
     t_fit = time.perf_counter() - t0
-    print(json.dumps({
-        "workload": f"colour fit, {cfg.name}, {a.views} views at {a.size}x{a.size}, "
-                    f"Adam lr {a.lr}, {a.steps} steps",
-        "splats": n, "visible_splats": int(seen.sum()), "steps": a.steps,
-        "ms_per_step": round(1e3 * t_fit / a.steps, 3),
-        "renders_per_s": round(a.steps * a.views / t_fit, 1),
+    out = {
+        "workload": f"colour fit, {cfg.name}, {views} views at {size}x{size}, "
+                    f"Adam lr {lr}, {steps} steps",
+        "splats": n, "visible_splats": int(seen.sum()), "steps": steps,
+        "ms_per_step": round(1e3 * t_fit / steps, 3),
+        "renders_per_s": round(steps * views / t_fit, 1),
         "loss_first": hist[0], "loss_last": hist[-1],
         "mean_abs_rgb_error_visible_init": round(float((init[seen] - gt[seen]).abs().mean()), 5),
         "mean_abs_rgb_error_visible": round(float((rgb[seen] - gt[seen]).abs().mean()), 5),
-        "setup_s": round(t_setup, 2)}))
+        "setup_s": round(t_setup, 2)}
+    del splats, vox, targets, bounds
+    return out
+
+
+def main(argv=None) -> int:
+    import argparse
+
+    ap = argparse.ArgumentParser(description="per-splat colour fit to rendered target views")
+    ap.add_argument("--views", type=int, default=64)
+    ap.add_argument("--size", type=int, default=512)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--lr", type=float, default=0.05)
+    a = ap.parse_args(argv)
+    gpu = GPU(0)
+    print(json.dumps(run_fit(gpu, a.views, a.size, a.steps, a.lr)))
     gpu.close()
     return 0
 

@@ -1,17 +1,20 @@
 """A resident world on one GPU: generate, edit, and run the per-frame chunk
-management pass -- batched remesh of the dirty chunks in one hipGraph launch,
-then RLE recompression of every other chunk.
+management pass -- batched remesh of the dirty chunks, then RLE
+recompression of every other chunk -- as one hipGraph captured once and
+replayed every frame.
 
 This is chunkset_manage's frame (src/chunkset.c:246-507) over a whole world
-held uncompressed in HBM: the reference meshes dirty chunks one per OpenMP
-thread (chunkset.c:251,339-487) and compresses chunks that went idle with
-rle_compress (chunkset.c:213-230); here each of the two is one launch over a
-list of chunks.  The edit burst is chunkset_edit_sphere (src/chunkset/edit.c:
-179-244): voxels on the GPU (vp_gpu_edit_spheres), the shadow-map updates of
-the add spheres in their sequential order on the host (vp_host_shadow_spheres,
-shadow.h:77-89).
+held uncompressed in HBM: the reference walks the chunks, meshes the dirty
+ones one per OpenMP thread (chunkset.c:251,339-487) and compresses the others
+with rle_compress (chunkset.c:213-230,300-306).  Here the walk is a device
+compaction of the dirty bytes into a dirty and an idle list, and each of the
+two passes is one launch that reads its list length from device memory
+(vp_host_frame_graph_*), so the same graph serves every edit burst.  The edit
+burst is chunkset_edit_sphere (src/chunkset/edit.c:179-244): voxels
+(vp_gpu_edit_spheres) and the shadow-map updates of the add spheres in their
+sequential order (vp_gpu_shadow_spheres, shadow.h:77-89), both on the device.
 
-    python -m voxplat_amd.world_frame [--frames K]     # 16384x512x16384, 10k edits
+    python -m voxplat_amd.world_frame [--frames K]     # 16384x512x16384, 10k edits per frame
 """
 from __future__ import annotations
 
@@ -25,7 +28,7 @@ import torch
 
 from . import abi
 from . import workload as W
-from ._lib import GPU, RemeshGraph, host_shadow_spheres
+from ._lib import GPU, FrameGraph, RemeshGraph
 
 MAP_NORESERVE = 0x4000          # Linux mmap flag: no swap reservation for the mapping
 
@@ -50,13 +53,10 @@ class ResidentWorld:
         self.gpu.shadow_build(self.w, self.voxels, self.smap)
 
     def edit(self, spheres) -> None:
-        """Apply the spheres in order; marks every chunk of each sphere's box dirty."""
+        """Apply the spheres in order; marks every chunk of each sphere's box
+        dirty.  The voxels and the shadow map are both updated on the device."""
         self.gpu.edit_spheres(self.w, self.voxels, spheres, self.dirty, self.uniform)
-        adds = [s for s in spheres if s.voxel]
-        if adds:
-            m = self.smap.cpu().numpy().view(np.uint16)
-            host_shadow_spheres(self.w, m, adds)
-            self.smap.copy_(torch.from_numpy(m.view(np.int16)))
+        self.gpu.shadow_spheres(self.w, self.smap, spheres)
 
     def open(self, ids, words, offsets, uniform) -> None:
         """Decode chunks `ids` from RLE streams back into the world.
@@ -99,48 +99,33 @@ def heaviest_first(gpu: GPU, w: abi.WorldDesc, voxels, ids, smap, uniform, lod: 
 
 
 class Frame:
-    """One frame of chunk management with every buffer sized up front: the
-    remesh of `dirty` is captured in a hipGraph and the RLE of `idle` reuses
-    its word buffer, so a frame allocates nothing."""
+    """The per-frame pass of a resident world, captured once (FrameGraph) and
+    replayed each frame: every launch rebuilds the dirty and idle lists from
+    the world's dirty bytes (clearing them), remeshes the dirty chunks and
+    RLE-encodes the others into buffers sized once for `record_capacity`
+    records and `word_capacity` words.  After `sync()` the attributes name
+    the last frame's lists and outputs."""
 
-    def __init__(self, world: ResidentWorld, dirty, idle, lod: bool = True):
-        gpu, w = world.gpu, world.w
-        if dirty.numel():
-            dirty, self.offsets, self.counts = heaviest_first(gpu, w, world.voxels, dirty,
-                                                              world.smap, world.uniform, lod)
-        else:
-            _, self.offsets, self.counts = gpu.cull_mesh(w, world.voxels, dirty,
-                                                         shadow_map=world.smap,
-                                                         uniform=world.uniform, lod=lod,
-                                                         count_only=True)
-        self.world, self.dirty, self.idle = world, dirty, idle
-        _, self.word_offsets, self.idle_uniform = gpu.rle_encode(w, world.voxels, idle,
-                                                                 count_only=True)
+    def __init__(self, world: ResidentWorld, record_capacity: int, word_capacity: int,
+                 lod: bool = True):
+        self.world = world
+        self.graph = FrameGraph(world.gpu, world.w, world.voxels, world.uniform, world.smap,
+                                world.dirty, record_capacity, word_capacity, lod=lod)
+        self.splats, self.words = self.graph.splats, self.graph.words
+
+    def run(self, stream=None) -> None:
+        self.graph.launch(stream)
+
+    def sync(self) -> None:
+        """Read the last frame's list lengths and expose its outputs."""
+        g = self.graph
         torch.cuda.synchronize()
-        self.records = int(self.offsets[-1].item()) // 4
-        self.words_n = int(self.word_offsets[-1].item()) if idle.numel() else 0
-        dev = world.voxels.device
-        self.splats = torch.empty((max(self.records, 1), 4), dtype=torch.int16, device=dev)
-        self.words = torch.empty(max(self.words_n, 1), dtype=torch.int32, device=dev)
-        gpu.reserve(max(int(dirty.numel()), int(idle.numel()), 1), 0)
-        self.graph = (RemeshGraph(gpu, w, world.voxels, dirty, world.smap, self.offsets,
-                                  self.counts, self.splats, self.records,
-                                  uniform=world.uniform, lod=lod)
-                      if dirty.numel() else None)
-
-    def remesh(self) -> None:
-        if self.graph is not None:
-            self.graph.launch()
-
-    def recompress(self) -> None:
-        if self.idle.numel():
-            self.world.gpu.rle_encode(self.world.w, self.world.voxels, self.idle,
-                                      capacity=self.words_n,
-                                      out=(self.words, self.word_offsets, self.idle_uniform))
-
-    def run(self) -> None:
-        self.remesh()
-        self.recompress()
+        self.dirty, self.idle = g.lists()
+        nd, ni = int(self.dirty.numel()), int(self.idle.numel())
+        self.offsets, self.counts = g.offsets[:nd + 1], g.level_counts[:nd]
+        self.word_offsets, self.idle_uniform = g.word_offsets[:ni + 1], g.idle_uniform[:ni]
+        self.records = int(self.offsets[-1].item()) // 4 if nd else 0
+        self.words_n = int(self.word_offsets[-1].item()) if ni else 0
 
     def chunk_records(self, k: int) -> torch.Tensor:
         """Records of dirty[k], levels 0..4 concatenated."""
@@ -265,26 +250,28 @@ def full_remesh(world: ResidentWorld, steps: int = 3, warmup: int = 1, log=None)
 def main(argv=None) -> int:
     import argparse
 
-    ap = argparse.ArgumentParser(description="resident-world frame: edits, remesh, RLE")
-    ap.add_argument("--frames", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap = argparse.ArgumentParser(description="resident-world frames: edit bursts, remesh, RLE")
+    ap.add_argument("--frames", type=int, default=4, help="edit bursts, one frame each")
     ap.add_argument("--edits", type=int, default=10000)
     ap.add_argument("--full-steps", type=int, default=0,
                     help="also time a remesh of every chunk (needs ~100 GB more HBM)")
     a = ap.parse_args(argv)
-    res = run_world16k(a.frames, a.warmup, a.edits, full_steps=a.full_steps,
+    res = run_world16k(a.frames, a.edits, full_steps=a.full_steps,
                        log=lambda m: print(m, file=sys.stderr, flush=True))
     print(json.dumps(res))
     return 0
 
 
-def run_world16k(frames: int = 5, warmup: int = 1, edits: int = 10000, log=None,
-                 keep: bool = False, gpu: GPU | None = None, full_steps: int = 0,
-                 on_generated=None):
-    """The 16384x512x16384 workload.  Returns a dict of timings (and, with
-    keep, the (world, frame, spheres) objects under "_objects").
-    on_generated(world), if given, runs after generation and the full
-    remesh, before the edits; its return value is kept under "on_generated"."""
+def run_world16k(frames: int = 4, edits: int = 10000, log=None, keep: bool = False,
+                 gpu: GPU | None = None, full_steps: int = 0, full_warmup: int = 1,
+                 on_generated=None, record_capacity: int = 1 << 31, word_margin: float = 1.25):
+    """The 16384x512x16384 workload.  Each frame is a new burst of `edits`
+    seeded sphere edits (seed EDIT_SEED + frame), then one replay of the frame
+    graph: lists, remesh of the dirty chunks, RLE of the others.  Returns a
+    dict of timings (and, with keep, (world, frame, spheres of every burst,
+    spheres of the last burst) under "_objects").  on_generated(world), if
+    given, runs after generation and the full remesh, before the edits; its
+    return value is kept under "on_generated"."""
     def say(m):
         if log:
             log(m)
@@ -300,64 +287,68 @@ def run_world16k(frames: int = 5, warmup: int = 1, edits: int = 10000, log=None,
     occ = {"uniform_air": int((uni_h == 0).sum()), "uniform_solid": int((uni_h > 0).sum()),
            "mixed": int((uni_h < 0).sum())}
     say(f"generated {cfg.name} in {t_gen:.2f} s, occupancy {occ}")
-    full = full_remesh(world, steps=full_steps, log=say) if full_steps else None
+    full = full_remesh(world, steps=full_steps, warmup=full_warmup, log=say) if full_steps else None
     if full:
         say(f"full remesh: {full}")
     extra = on_generated(world) if on_generated else None
-    spheres = W.edit_burst(world.w, n=edits)
-    t0 = time.perf_counter()
-    world.edit(spheres)
+    # word budget: every chunk's stream now, with a margin for the edits
+    ids = torch.arange(world.w.chunks, dtype=torch.int32, device=world.voxels.device)
+    _, woffs, _ = gpu.rle_encode(world.w, world.voxels, ids, count_only=True)
     torch.cuda.synchronize()
-    t_edit = time.perf_counter() - t0
-    dirty, idle = world.lists()
-    say(f"{edits} edits in {t_edit:.2f} s: {dirty.numel()} dirty, {idle.numel()} idle chunks")
-    fr = Frame(world, dirty, idle)
-    say(f"frame buffers: {fr.records} records, {fr.words_n} RLE words")
+    word_capacity = int(int(woffs[-1].item()) * word_margin)
+    del ids, woffs
+    fr = Frame(world, record_capacity, word_capacity)
+    say(f"frame graph: {record_capacity} records, {word_capacity} words")
     st = torch.cuda.current_stream()
-    for _ in range(warmup):
-        fr.run()
-    torch.cuda.synchronize()
-    ev = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
-    rm = rl = 0.0
-    tw = time.perf_counter()
-    for _ in range(frames):
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+    per = []
+    every, last = [], []
+    for f in range(frames):
+        spheres = W.edit_burst(world.w, n=edits, seed=W.EDIT_SEED + f)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        world.edit(spheres)
+        torch.cuda.synchronize()
+        t_edit = time.perf_counter() - t0
+        tw = time.perf_counter()
         ev[0].record(st)
-        fr.remesh()
+        fr.run()
         ev[1].record(st)
-        fr.recompress()
-        ev[2].record(st)
-        ev[2].synchronize()
-        rm += ev[0].elapsed_time(ev[1])
-        rl += ev[1].elapsed_time(ev[2])
-    wall = (time.perf_counter() - tw) * 1e3 / frames
-    rm, rl = rm / frames, rl / frames
-    # reopen: every idle chunk decoded back in place from the stream just
-    # written (the bytes it writes are the bytes already there)
+        ev[1].synchronize()
+        wall = (time.perf_counter() - tw) * 1e3
+        ms = ev[0].elapsed_time(ev[1])
+        fr.sync()
+        status = gpu.status()
+        nd, ni = int(fr.dirty.numel()), int(fr.idle.numel())
+        dirty_uni = world.uniform[fr.dirty.long()].cpu().numpy()
+        rb = remesh_bytes(dirty_uni, fr.counts.cpu().numpy().view(np.uint32))
+        lb = rle_bytes(ni, fr.words_n, world.R3)
+        per.append({"edit_s": round(t_edit, 3), "dirty_chunks": nd, "idle_chunks": ni,
+                    "records": fr.records, "rle_words": fr.words_n,
+                    "ms": round(max(ms, wall), 3), "graph_ms": round(ms, 3),
+                    "algorithmic_bytes": rb + lb,
+                    "gb_s": round((rb + lb) / (ms * 1e-3) / 1e9, 1), "status": status})
+        say(f"frame {f}: {per[-1]}")
+        every.extend(spheres)
+        last = spheres
+    # reopen: every idle chunk of the last frame decoded back in place from
+    # the stream just written (the bytes it writes are the bytes already there)
     ev[0].record(st)
-    world.open(idle, fr.words, fr.word_offsets, fr.idle_uniform)
+    world.open(fr.idle, fr.words, fr.word_offsets, fr.idle_uniform)
     ev[1].record(st)
     ev[1].synchronize()
     ro = ev[0].elapsed_time(ev[1])
     status = gpu.status()
-    nd, ni = int(dirty.numel()), int(idle.numel())
-    dirty_uni = world.uniform[dirty.long()].cpu().numpy()
-    counts_h = fr.counts.cpu().numpy().view(np.uint32)
-    rb = remesh_bytes(dirty_uni, counts_h)
-    lb = rle_bytes(ni, fr.words_n, world.R3)
+    lb = rle_bytes(int(fr.idle.numel()), fr.words_n, world.R3)
+    mean = lambda k: round(float(np.mean([p[k] for p in per])), 3)
     out = {
-        "workload": f"{cfg.name} ({world.w.chunks} chunks of 64^3, 128 GiB in HBM), "
-                    f"{edits} sphere edits r 1-32, then per frame: batched remesh L0-L4 of the "
-                    "dirty chunks (one hipGraph launch) + RLE encode of the other chunks",
-        "occupancy": occ, "generate_s": round(t_gen, 3), "edit_s": round(t_edit, 3),
-        "dirty_chunks": nd, "idle_chunks": ni, "records": fr.records, "rle_words": fr.words_n,
-        "frames": frames, "ms_per_frame": round(max(wall, rm + rl), 3),
-        "remesh_ms": round(rm, 3),
-        "remesh_gvoxels_s": round(nd * world.R3 / (rm * 1e-3) / 1e9, 2) if rm else None,
-        "remesh_gb_s": round(rb / (rm * 1e-3) / 1e9, 1) if rm else None,
-        "rle_ms": round(rl, 3),
-        "rle_gvoxels_s": round(ni * world.R3 / (rl * 1e-3) / 1e9, 2) if rl else None,
-        "rle_gb_s": round(lb / (rl * 1e-3) / 1e9, 1) if rl else None,
-        "frame_gb_s": round((rb + lb) / ((rm + rl) * 1e-3) / 1e9, 1),
+        "workload": f"{cfg.name} ({world.w.chunks} chunks of 64^3, 128 GiB in HBM); {frames} "
+                    f"frames, each: {edits} sphere edits r 1-32 (voxels and shadow map on the "
+                    "device), then one replay of a graph captured once: dirty/idle lists on the "
+                    "device, batched remesh L0-L4 of the dirty chunks, RLE encode of the others",
+        "occupancy": occ, "generate_s": round(t_gen, 3),
+        "frames": per, "ms_per_frame": mean("ms"), "edit_s": mean("edit_s"),
+        "frame_gb_s": mean("gb_s"),
         "reopen_ms": round(ro, 3),
         "reopen_gb_s": round(lb / (ro * 1e-3) / 1e9, 1) if ro else None,
         "status": status,
@@ -365,7 +356,7 @@ def run_world16k(frames: int = 5, warmup: int = 1, edits: int = 10000, log=None,
         "on_generated": extra,
     }
     if keep:
-        out["_objects"] = (world, fr, spheres)
+        out["_objects"] = (world, fr, every, last)
     else:
         fr.close()
         del world
