@@ -343,6 +343,43 @@ int vp_gpu_draw_gather(vp_ctx *ctx, const vp_world_desc *world,
                        uint32_t n, void *d_out, uint8_t *d_out_lod,
                        uint64_t capacity_records, void *stream);
 
+/* The frame's draw queue selected on the device: octree_cull
+ * (src/game.c:319-395) from every level-4 node (game.c:507-512), with
+ * gfx_fcull_visible's bounding-sphere test (src/gfx/fcull.c:38-45), exactly as
+ * vp_host_octree_cull (include/vp_host.h) selects it, draw for draw: queue
+ * order, levels, nodes and running `first` record offsets.  d_node_offsets[l]:
+ * device node offsets of level l (vp_gpu_svl_nodes); d_chunk_mesh (nullable,
+ * one byte per chunk): level-0 chunks with a quad mesh go to d_mesh_chunks.
+ * d_counts[0..2] receives (draws, mesh draws, records) in device memory; the
+ * host reads nothing, so gather and splat can follow on the stream with
+ * vp_gpu_draw_gather_counted and vp_gpu_splat_counted.  Draws or mesh chunks
+ * past their capacity are not written (their counts are) and
+ * VP_STATUS_OVERFLOW is raised.  Needs sum(max_bitw) <= 27. */
+int vp_gpu_octree_cull(vp_ctx *ctx, const vp_world_desc *world, const float planes[24],
+                       const float origin[3], const uint64_t *const d_node_offsets[5],
+                       const uint8_t *d_chunk_mesh, vp_draw *d_draws, uint32_t draw_cap,
+                       uint32_t *d_mesh_chunks, uint32_t mesh_cap, uint64_t *d_counts,
+                       void *stream);
+
+/* vp_gpu_draw_gather with the draw count read from device memory:
+ * min(d_counts[0], draw_cap) draws of d_draws (d_counts as vp_gpu_octree_cull
+ * writes it). */
+int vp_gpu_draw_gather_counted(vp_ctx *ctx, const vp_world_desc *world,
+                               const void *const d_node_splats[5],
+                               const uint64_t *const d_node_offsets[5], const vp_draw *d_draws,
+                               uint32_t draw_cap, const uint64_t *d_counts, void *d_out,
+                               uint8_t *d_out_lod, uint64_t capacity_records, void *stream);
+
+/* vp_gpu_splat_bounds + vp_gpu_splat_blocks for a list whose length is in
+ * device memory: the first min(*d_count, capacity) splats.  d_bounds holds
+ * ceil(capacity / VP_SPLAT_BLOCK) * 8 floats of workspace; the bounds are
+ * computed into it, then the frame is resolved.  The result equals
+ * vp_gpu_splat's on that many splats. */
+int vp_gpu_splat_counted(vp_ctx *ctx, const int16_t *d_splats, const uint8_t *d_lod,
+                         uint32_t lod_log2, uint64_t capacity, const uint64_t *d_count,
+                         float *d_bounds, const vp_camera *cam, const float *d_voxel_rgb,
+                         const vp_frame *out, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -462,6 +462,71 @@ class GPU:
                "vp_gpu_draw_gather")
         return recs, lod
 
+    def octree_cull(self, world: abi.WorldDesc, planes, origin, node_offsets, chunk_mesh=None,
+                    draw_cap: int = 65536, mesh_cap: int = 65536, out=None, stream=None):
+        """vp_gpu_octree_cull: the frame's draw queue selected on the device.
+        node_offsets: the five levels' device node offsets.  Returns (draws
+        int64[draw_cap, 2] (vp_draw {u32 level, u32 node, u64 first}), mesh chunk
+        ids int32[mesh_cap], counts int64[3] = draws, mesh draws, records)."""
+        import numpy as np
+        torch = self.torch
+        dev = node_offsets[0].device
+        if out is None:
+            draws = torch.empty((max(draw_cap, 1), 2), dtype=torch.int64, device=dev)
+            mesh = torch.empty(max(mesh_cap, 1), dtype=torch.int32, device=dev)
+            counts = torch.empty(3, dtype=torch.int64, device=dev)
+        else:
+            draws, mesh, counts = out
+        for l in range(5):
+            _need(node_offsets[l], (abi.node_count(world, l) + 1) * 8, "node offsets")
+        _need(chunk_mesh, world.chunks, "chunk_mesh")
+        pl = np.ascontiguousarray(planes, np.float32)
+        org = np.ascontiguousarray(origin, np.float32)
+        optr = (C.c_void_p * 5)(*[o.data_ptr() for o in node_offsets])
+        _check(self.lib.vp_gpu_octree_cull(
+            self.ctx, C.byref(world), pl.ctypes.data_as(C.POINTER(C.c_float)),
+            org.ctypes.data_as(C.POINTER(C.c_float)), optr, C.c_void_p(_ptr(chunk_mesh)),
+            C.c_void_p(draws.data_ptr()), draw_cap, C.c_void_p(mesh.data_ptr()), mesh_cap,
+            C.c_void_p(counts.data_ptr()), C.c_void_p(self._stream(stream))), "vp_gpu_octree_cull")
+        return draws, mesh, counts
+
+    def draw_frame(self, world: abi.WorldDesc, node_splats, node_offsets, planes, origin,
+                   cam: abi.Camera, draw_cap: int, record_capacity: int, chunk_mesh=None,
+                   stream=None):
+        """The LOD-selected frame with no host read between its steps: the draw
+        queue on the device (vp_gpu_octree_cull), the assembled list
+        (vp_gpu_draw_gather_counted) and its block-culled resolve
+        (vp_gpu_splat_counted).  Returns (depth, ids, records drawn); the record
+        count is read only after the frame is queued."""
+        torch = self.torch
+        dev = node_offsets[0].device
+        draws, _, counts = self.octree_cull(world, planes, origin, node_offsets,
+                                            chunk_mesh=chunk_mesh, draw_cap=draw_cap,
+                                            stream=stream)
+        lst = torch.empty((max(record_capacity, 1), 4), dtype=torch.int16, device=dev)
+        lod = torch.empty(max(record_capacity, 1), dtype=torch.uint8, device=dev)
+        st = C.c_void_p(self._stream(stream))
+        nptr = (C.c_void_p * 5)(*[n.data_ptr() for n in node_splats])
+        optr = (C.c_void_p * 5)(*[o.data_ptr() for o in node_offsets])
+        _check(self.lib.vp_gpu_draw_gather_counted(
+            self.ctx, C.byref(world), nptr, optr, C.c_void_p(draws.data_ptr()), draw_cap,
+            C.c_void_p(counts.data_ptr()), C.c_void_p(lst.data_ptr()), C.c_void_p(lod.data_ptr()),
+            record_capacity, st), "vp_gpu_draw_gather_counted")
+        nb = (record_capacity + abi.VP_SPLAT_BLOCK - 1) // abi.VP_SPLAT_BLOCK
+        bounds = torch.empty((max(nb, 1), 8), dtype=torch.float32, device=dev)
+        depth = torch.empty((cam.height, cam.width), dtype=torch.float32, device=dev)
+        ids = torch.empty((cam.height, cam.width), dtype=torch.int32, device=dev)
+        fr = abi.Frame()
+        fr.d_depth, fr.d_id, fr.d_rgb, fr.d_weight = depth.data_ptr(), ids.data_ptr(), None, None
+        # the records drawn are the queue's running total: counts[2]
+        total_ptr = counts.data_ptr() + 16
+        _check(self.lib.vp_gpu_splat_counted(
+            self.ctx, C.c_void_p(lst.data_ptr()), C.c_void_p(lod.data_ptr()), 0, record_capacity,
+            C.c_void_p(total_ptr), C.c_void_p(bounds.data_ptr()), C.byref(cam), None,
+            C.byref(fr), st), "vp_gpu_splat_counted")
+        torch.cuda.synchronize()
+        return depth, ids, int(counts[2].item())
+
     def generate(self, world: abi.WorldDesc, params: abi.GenParams, voxels, uniform=None,
                  chunk0: int = 0, count: int | None = None, stream=None):
         count = world.chunks - chunk0 if count is None else count

@@ -148,6 +148,19 @@ PROTOTYPES = {
 VP_SPLAT_BLOCK = 256
 
 
+PROTOTYPES.update({
+    "vp_gpu_octree_cull": (C.c_int, [C.c_void_p, C.POINTER(WorldDesc), C.POINTER(C.c_float),
+                                     C.POINTER(C.c_float), C.c_void_p, C.c_void_p, C.c_void_p,
+                                     C.c_uint32, C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p]),
+    "vp_gpu_draw_gather_counted": (C.c_int, [C.c_void_p, C.POINTER(WorldDesc), C.c_void_p,
+                                             C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p,
+                                             C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p]),
+    "vp_gpu_splat_counted": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32,
+                                       C.c_uint64, C.c_void_p, C.c_void_p, C.POINTER(Camera),
+                                       C.c_void_p, C.POINTER(Frame), C.c_void_p]),
+})
+
+
 # ---- include/vp_workload.h ---------------------------------------------------
 class GenParams(C.Structure):
     _fields_ = [("seed", C.c_uint64), ("period", C.c_uint32), ("hmin", C.c_int32),

@@ -248,6 +248,7 @@ struct GatherArgs {
     uint64_t ncount[5];
     const vp_draw *draws;
     uint32_t n;
+    const uint64_t *d_n;        // nullable: the draw count in device memory, n its bound
     uint64_t *out;
     uint8_t *lod;
     uint64_t cap;
@@ -260,7 +261,8 @@ struct GatherArgs {
 __global__ void __launch_bounds__(256) k_draw_gather(GatherArgs a)
 {
     const uint64_t ns = gridDim.y, sl = blockIdx.y;
-    for (uint32_t d = blockIdx.x; d < a.n; d += gridDim.x) {
+    const uint32_t n = a.d_n ? (uint32_t)min(*a.d_n, (uint64_t)a.n) : a.n;
+    for (uint32_t d = blockIdx.x; d < n; d += gridDim.x) {
         const uint32_t l = a.draws[d].level, node = a.draws[d].node;
         const uint64_t first = a.draws[d].first;
         if (l >= 5 || node >= a.ncount[l] || !a.nodes[l] || !a.noff[l]) {
@@ -387,11 +389,10 @@ extern "C" int vp_gpu_svl_nodes_all(vp_ctx *ctx, const vp_world_desc *world, con
                             d_level_counts, ordered, stream);
 }
 
-extern "C" int vp_gpu_draw_gather(vp_ctx *ctx, const vp_world_desc *world,
-                                  const void *const d_node_splats[5],
-                                  const uint64_t *const d_node_offsets[5], const vp_draw *d_draws,
-                                  uint32_t n, void *d_out, uint8_t *d_out_lod,
-                                  uint64_t capacity_records, void *stream)
+static int draw_gather(vp_ctx *ctx, const vp_world_desc *world, const void *const d_node_splats[5],
+                       const uint64_t *const d_node_offsets[5], const vp_draw *d_draws, uint32_t n,
+                       const uint64_t *d_n, void *d_out, uint8_t *d_out_lod,
+                       uint64_t capacity_records, void *stream)
 {
     if (!ctx || !world || !d_node_splats || !d_node_offsets) return VP_ERR_ARG;
     if (n == 0) return VP_OK;
@@ -404,6 +405,7 @@ extern "C" int vp_gpu_draw_gather(vp_ctx *ctx, const vp_world_desc *world,
     }
     a.draws = d_draws;
     a.n = n;
+    a.d_n = d_n;
     a.out = static_cast<uint64_t *>(d_out);
     a.lod = d_out_lod;
     a.cap = capacity_records;
@@ -411,4 +413,27 @@ extern "C" int vp_gpu_draw_gather(vp_ctx *ctx, const vp_world_desc *world,
     const int grid = (int)(n < 65535u ? n : 65535u);
     hipLaunchKernelGGL(k_draw_gather, dim3(grid, kGatherSlices), dim3(256), 0, as_stream(stream), a);
     return hipGetLastError() == hipSuccess ? VP_OK : VP_ERR_HIP;
+}
+
+extern "C" int vp_gpu_draw_gather(vp_ctx *ctx, const vp_world_desc *world,
+                                  const void *const d_node_splats[5],
+                                  const uint64_t *const d_node_offsets[5], const vp_draw *d_draws,
+                                  uint32_t n, void *d_out, uint8_t *d_out_lod,
+                                  uint64_t capacity_records, void *stream)
+{
+    return draw_gather(ctx, world, d_node_splats, d_node_offsets, d_draws, n, nullptr, d_out,
+                       d_out_lod, capacity_records, stream);
+}
+
+extern "C" int vp_gpu_draw_gather_counted(vp_ctx *ctx, const vp_world_desc *world,
+                                          const void *const d_node_splats[5],
+                                          const uint64_t *const d_node_offsets[5],
+                                          const vp_draw *d_draws, uint32_t draw_cap,
+                                          const uint64_t *d_counts, void *d_out,
+                                          uint8_t *d_out_lod, uint64_t capacity_records,
+                                          void *stream)
+{
+    if (!d_counts) return VP_ERR_ARG;
+    return draw_gather(ctx, world, d_node_splats, d_node_offsets, d_draws, draw_cap, d_counts,
+                       d_out, d_out_lod, capacity_records, stream);
 }

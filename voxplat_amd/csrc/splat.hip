@@ -257,9 +257,13 @@ constexpr int kCounters = kTotal + 1;
 // blocks strided over the grid's waves; a wave-wide min reduction, no LDS.
 __global__ void __launch_bounds__(256) k_bounds(const int16_t *splats, const uint8_t *lod,
                                                 uint32_t lod_log2, uint64_t n, uint64_t nb,
-                                                float *bounds)
+                                                float *bounds, const uint64_t *d_n)
 {
     constexpr int PER = kBlock / 64;
+    if (d_n) {          // the list length lives in device memory (n is its bound)
+        n = min(*d_n, n);
+        nb = (n + kBlock - 1) / kBlock;
+    }
     const uint32_t lane = threadIdx.x & 63u;
     const uint64_t nwaves = (uint64_t)gridDim.x * (blockDim.x >> 6);
     for (uint64_t b = (uint64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); b < nb;
@@ -338,9 +342,10 @@ __device__ __forceinline__ uint32_t block_bin(const float *bounds, uint64_t b,
 // Bins of the blocks, counted per workgroup in LDS and added to the global
 // counts once per workgroup and bin.
 __global__ void __launch_bounds__(256) k_blk_cull(const float *bounds, uint64_t nb, CullPlanes cp,
-                                                  uint32_t *bin, uint32_t *cnt)
+                                                  uint32_t *bin, uint32_t *cnt, const uint64_t *d_n)
 {
     __shared__ uint32_t hist[kBins];
+    if (d_n) nb = min((*d_n + kBlock - 1) / kBlock, nb);
     for (int k = threadIdx.x; k < kBins; k += blockDim.x) hist[k] = 0u;
     __syncthreads();
     for (uint64_t b = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; b < nb;
@@ -375,9 +380,11 @@ __global__ void __launch_bounds__(kBins) k_blk_scan(uint32_t *cnt)
 // counts its blocks per bin in LDS, reserves each bin's share with one global
 // add, then places every block inside its share.
 __global__ void __launch_bounds__(256) k_blk_scatter(const uint32_t *bin, uint64_t nb,
-                                                     uint32_t *cnt, uint32_t *order)
+                                                     uint32_t *cnt, uint32_t *order,
+                                                     const uint64_t *d_n)
 {
     __shared__ uint32_t num[kBins], base[kBins];
+    if (d_n) nb = min((*d_n + kBlock - 1) / kBlock, nb);
     for (uint64_t b0 = blockIdx.x * (uint64_t)blockDim.x; b0 < nb;
          b0 += (uint64_t)gridDim.x * blockDim.x) {
         const uint64_t b = b0 + threadIdx.x;
@@ -403,9 +410,11 @@ __global__ void __launch_bounds__(kBlock) k_raster_blocks(const int16_t *splats,
                                                            uint64_t n, SplatCam c,
                                                            const float4 *rays, uint64_t *frame,
                                                            const uint32_t *order,
-                                                           const uint32_t *cnt)
+                                                           const uint32_t *cnt,
+                                                           const uint64_t *d_n)
 {
     const uint32_t total = cnt[kTotal];
+    if (d_n) n = min(*d_n, n);
     for (uint32_t t = blockIdx.x; t < total; t += gridDim.x) {
         const uint64_t i = (uint64_t)order[t] * kBlock + threadIdx.x;
         raster_splat(splats, lod, lod_log2, n, i, c, rays, frame);
@@ -535,10 +544,13 @@ static CullPlanes cull_planes(const vp_camera *cam)
     return cp;
 }
 
+// The splat frame.  d_n (nullable): the list length in device memory, n its
+// bound; with d_n the block bounds are computed into d_bounds here first (the
+// list is assembled on the device each frame).
 static int splat_frame(vp_ctx *ctx, const int16_t *d_splats, const uint8_t *d_lod,
                        uint32_t lod_log2, uint64_t n, const float *d_bounds,
                        const vp_camera *cam, const float *d_voxel_rgb, const vp_frame *out,
-                       void *stream)
+                       void *stream, const uint64_t *d_n = nullptr)
 {
     if (!ctx || !cam || !out || !out->d_depth || !out->d_id) return VP_ERR_ARG;
     if (n && !d_splats) return VP_ERR_ARG;
@@ -570,17 +582,22 @@ static int splat_frame(vp_ctx *ctx, const int16_t *d_splats, const uint8_t *d_lo
     const dim3 pg((cam->width + 255) / 256, cam->height);
     hipLaunchKernelGGL(k_rays, pg, dim3(256), 0, st, c, rays);
     hipLaunchKernelGGL(k_clear, dim3(1024), dim3(256), 0, st, frame, np);
+    if (n && blk && d_n) {
+        const uint64_t wgs = (nb + 3) / 4, cap = (uint64_t)ncu * 32u;
+        hipLaunchKernelGGL(k_bounds, dim3((uint32_t)(wgs < cap ? wgs : cap)), dim3(256), 0, st,
+                           d_splats, d_lod, lod_log2, n, nb, const_cast<float *>(d_bounds), d_n);
+    }
     if (n && blk) {
         // blk: [bin per block | order | counters]
         uint32_t *bin = blk, *order = blk + nb, *cnt = blk + 2 * nb;
         const uint32_t g = (uint32_t)((nb + 255) / 256 < 4096 ? (nb + 255) / 256 : 4096);
         hipLaunchKernelGGL(k_blk_reset, dim3(1), dim3(256), 0, st, cnt);
         hipLaunchKernelGGL(k_blk_cull, dim3(g), dim3(256), 0, st, d_bounds, nb, cull_planes(cam),
-                           bin, cnt);
+                           bin, cnt, d_n);
         hipLaunchKernelGGL(k_blk_scan, dim3(1), dim3(kBins), 0, st, cnt);
-        hipLaunchKernelGGL(k_blk_scatter, dim3(g), dim3(256), 0, st, bin, nb, cnt, order);
+        hipLaunchKernelGGL(k_blk_scatter, dim3(g), dim3(256), 0, st, bin, nb, cnt, order, d_n);
         hipLaunchKernelGGL(k_raster_blocks, dim3((uint32_t)ncu * 8u), dim3(kBlock), 0, st,
-                           d_splats, d_lod, lod_log2, n, c, rays, frame, order, cnt);
+                           d_splats, d_lod, lod_log2, n, c, rays, frame, order, cnt, d_n);
     } else if (n) {
         hipLaunchKernelGGL(k_raster, dim3((uint32_t)nb), dim3(256), 0, st, d_splats, d_lod,
                            lod_log2, n, c, rays, frame);
@@ -610,7 +627,7 @@ extern "C" int vp_gpu_splat_bounds(vp_ctx *ctx, const int16_t *d_splats, const u
     // four blocks per workgroup (a wave each), at most 32 workgroups per CU
     const uint64_t wgs = (nb + 3) / 4, cap = (uint64_t)ctx->num_cus * 32u;
     hipLaunchKernelGGL(k_bounds, dim3((uint32_t)(wgs < cap ? wgs : cap)), dim3(256), 0,
-                       as_stream(stream), d_splats, d_lod, lod_log2, n, nb, d_bounds);
+                       as_stream(stream), d_splats, d_lod, lod_log2, n, nb, d_bounds, nullptr);
     VP_HIP_TRY(hipGetLastError());
     return VP_OK;
 }
@@ -636,4 +653,14 @@ extern "C" int vp_gpu_splat_backward(vp_ctx *ctx, uint64_t n, const vp_camera *c
                        d_id, d_weight, d_dL_drgb, d_grad);
     VP_HIP_TRY(hipGetLastError());
     return VP_OK;
+}
+
+extern "C" int vp_gpu_splat_counted(vp_ctx *ctx, const int16_t *d_splats, const uint8_t *d_lod,
+                                    uint32_t lod_log2, uint64_t capacity, const uint64_t *d_count,
+                                    float *d_bounds, const vp_camera *cam,
+                                    const float *d_voxel_rgb, const vp_frame *out, void *stream)
+{
+    if (!d_count || (capacity && !d_bounds)) return VP_ERR_ARG;
+    return splat_frame(ctx, d_splats, d_lod, lod_log2, capacity, d_bounds, cam, d_voxel_rgb, out,
+                       stream, d_count);
 }

@@ -82,6 +82,37 @@ int ensure_cmap(vp_ctx *ctx, uint64_t chunks)
     return VP_OK;
 }
 
+int ensure_oct(vp_ctx *ctx, uint64_t slots, uint64_t blocks)
+{
+    if (slots > ctx->oct_cap) {
+        uint32_t *p = nullptr;
+        if (hipMalloc(&p, slots * sizeof(uint32_t)) != hipSuccess) return VP_ERR_NOMEM;
+        if (ctx->d_oct) {
+            if (hipDeviceSynchronize() != hipSuccess) {
+                (void)hipFree(p);
+                return VP_ERR_HIP;
+            }
+            (void)hipFree(ctx->d_oct);
+        }
+        ctx->d_oct = p;
+        ctx->oct_cap = slots;
+    }
+    if (3 * blocks > ctx->oct_part_cap) {
+        uint64_t *p = nullptr;
+        if (hipMalloc(&p, 3 * blocks * sizeof(uint64_t)) != hipSuccess) return VP_ERR_NOMEM;
+        if (ctx->d_oct_part) {
+            if (hipDeviceSynchronize() != hipSuccess) {
+                (void)hipFree(p);
+                return VP_ERR_HIP;
+            }
+            (void)hipFree(ctx->d_oct_part);
+        }
+        ctx->d_oct_part = p;
+        ctx->oct_part_cap = 3 * blocks;
+    }
+    return VP_OK;
+}
+
 }  // namespace vp
 
 extern "C" int vp_ctx_create(int device, vp_ctx **out)
@@ -131,6 +162,8 @@ extern "C" int vp_ctx_destroy(vp_ctx *ctx)
     if (ctx->d_rays) (void)hipFree(ctx->d_rays);
     if (ctx->d_blk) (void)hipFree(ctx->d_blk);
     if (ctx->d_cmap) (void)hipFree(ctx->d_cmap);
+    if (ctx->d_oct) (void)hipFree(ctx->d_oct);
+    if (ctx->d_oct_part) (void)hipFree(ctx->d_oct_part);
     delete ctx;
     return VP_OK;
 }

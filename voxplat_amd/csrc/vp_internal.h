@@ -32,6 +32,10 @@ struct vp_ctx {
     uint64_t  rays_cap = 0;
     uint32_t *d_cmap = nullptr;        // draw lists: list entry + 1 per chunk index
     uint64_t  cmap_cap = 0;
+    uint32_t *d_oct = nullptr;         // device draw-queue selection: key slots
+    uint64_t  oct_cap = 0;
+    uint64_t *d_oct_part = nullptr;    // and per-workgroup partial counts
+    uint64_t  oct_part_cap = 0;
     // device address of the static scratch of vp_gpu_cull_mesh_noalloc on
     // this context's device (noalloc.hip); not owned
     uint64_t *d_noalloc = nullptr;
@@ -46,6 +50,7 @@ int ensure_state(vp_ctx *ctx, uint64_t words);
 int ensure_frame(vp_ctx *ctx, uint64_t pixels);
 int ensure_blocks(vp_ctx *ctx, uint64_t blocks);
 int ensure_cmap(vp_ctx *ctx, uint64_t chunks);
+int ensure_oct(vp_ctx *ctx, uint64_t slots, uint64_t blocks);
 // fetch the static scratch's device address on the current device
 int noalloc_init(vp_ctx *ctx);
 
