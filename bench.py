@@ -98,7 +98,8 @@ def leg_world16k(args) -> dict:
         return res
 
     res = WF.run_world16k(frames=args.frames, edits=10000, log=log, full_steps=args.steps,
-                          full_warmup=args.warmup, on_generated=cpu_leg)
+                          full_warmup=args.warmup, on_generated=cpu_leg,
+                          autotune=not args.no_autotune)
     full = res["full_remesh"]
     gvox = full["gvoxels_s"]
     return {
@@ -111,7 +112,7 @@ def leg_world16k(args) -> dict:
                                "128 GiB resident), batched persistent kernel in a hipGraph, "
                                "list heaviest first",
                    "chunks": full["chunks"], "occupancy": res["occupancy"],
-                   "splat_records": full["records"]},
+                   "splat_records": full["records"], "launch": full["launch"]},
         "roofline": {"bound": "hbm", "achieved": full["gb_s"], "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(full["gb_s"] / HBM_PEAK_GBS, 4),
                      "traffic": measured_traffic("16384x512x16384"),
@@ -162,10 +163,12 @@ def leg_world2k(args) -> dict:
 
     tune = None
     if not args.no_autotune:
-        b, wg, ms = gpu.autotune_remesh(w, vox, ids, smap, splats, cap, offs, counts,
-                                        uniform=uni, reps=3)
-        tune = {"block": b, "wg_per_cu": wg, "ms": round(ms, 4)}
-        log(f"autotune: block {b} x {wg}/CU -> {ms:.3f} ms")
+        tune = gpu.autotune_remesh(w, vox, ids, smap, splats, cap, offs, counts, uniform=uni,
+                                   reps=3)
+        tune["ms"] = round(tune["ms"], 4)
+        log(f"autotune ({tune['shapes_timed']} resident shapes): block {tune['block']} x "
+            f"{tune['wg_per_cu']}/CU, {tune['records_in_flight']} records in flight -> "
+            f"{tune['ms']:.3f} ms")
     graph = RemeshGraph(gpu, w, vox, ids, smap, offs, counts, splats, cap, uniform=uni)
 
     # ---- batched, graph replay: the headline ----

@@ -269,6 +269,18 @@ int vp_gpu_splat_blocks(vp_ctx *ctx, const int16_t *d_splats, const uint8_t *d_l
                         const vp_camera *cam, const float *d_voxel_rgb,
                         const vp_frame *out, void *stream);
 
+/* How vp_gpu_splat_blocks and vp_gpu_splat_counted resolve a frame of at
+ * most 8192 screen tiles of 32x32 pixels (up to 3840x2160 and a little more).
+ * With enable = 1 the visible splats of at most 8x8 pixels are binned into
+ * the tiles they cover and each tile is resolved in on-chip memory, after the
+ * larger ones are drawn against the frame in device memory; frames with more
+ * than max_entries (tile, splat) entries (0: 2^26, about 256 MiB of context
+ * memory) fall back to rasterising every splat against the frame in device
+ * memory, as do all frames with enable = 0 (the default).  Both give the same
+ * frame bit for bit; this sets memory and speed only.  No counterpart in the
+ * reference (the GL rasteriser bins fragments in hardware). */
+int vp_ctx_set_splat_tiles(vp_ctx *ctx, int enable, uint64_t max_entries);
+
 /* dL/d voxel_rgb += sum over pixels p with id(p)=i of weight(p) * dL/drgb(p).
  * d_id/d_weight are the forward's outputs; d_grad (n*3) is accumulated. */
 int vp_gpu_splat_backward(vp_ctx *ctx, uint64_t n, const vp_camera *cam,

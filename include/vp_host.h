@@ -22,7 +22,25 @@ extern "C" {
 int vp_ctx_reserve(vp_ctx *ctx, uint64_t max_chunks, uint64_t max_pixels);
 
 /* Launch shape of the persistent remesh kernel: threads per workgroup (256,
- * 512 or 1024) and resident workgroups per CU (1..4). */
+ * 512 or 1024), workgroups per CU the grid is sized for (1..4), and the
+ * records each lane prepares before it stores them (2 or 4; chunks of 64^3
+ * only, smaller chunks always use 4).  No counterpart in the reference, which
+ * meshes on the CPU.  A world without a shape of its own (set for its
+ * root_bitw and max_bitw) uses the default (world = NULL): 512, 2, 4. */
+typedef struct {
+    int block;
+    int wg_per_cu;
+    int records_in_flight;
+} vp_remesh_shape;
+int vp_ctx_set_remesh_shape(vp_ctx *ctx, const vp_world_desc *world,
+                            const vp_remesh_shape *shape);
+int vp_ctx_get_remesh_shape(vp_ctx *ctx, const vp_world_desc *world, vp_remesh_shape *shape);
+/* Workgroups of the shape's kernel (for the world's chunk size and the
+ * record flags) that fit on one CU at once, from its registers and LDS;
+ * wg_per_cu above it only queues workgroups. */
+int vp_ctx_remesh_resident(vp_ctx *ctx, const vp_world_desc *world, uint32_t flags,
+                           const vp_remesh_shape *shape, int *per_cu);
+/* The default shape's block and wg_per_cu. */
 int vp_ctx_set_launch(vp_ctx *ctx, int block, int wg_per_cu);
 int vp_ctx_get_launch(vp_ctx *ctx, int *block, int *wg_per_cu);
 
@@ -93,14 +111,18 @@ int vp_host_frame_graph_create(vp_ctx *ctx, const vp_world_desc *world, uint8_t 
 int vp_host_frame_graph_launch(vp_frame_graph *graph, void *stream);
 int vp_host_frame_graph_destroy(vp_frame_graph *graph);
 
-/* Autotuner: time every launch shape on the given workload (reps runs each,
- * median), keep the fastest in the context.  Returns it and its time. */
+/* Autotuner: time every launch shape whose workgroups are all resident
+ * (vp_ctx_remesh_resident) on the given workload -- block 256, 512, 1024,
+ * wg_per_cu 1 up to the resident count, 2 and 4 records in flight for 64^3
+ * chunks -- reps runs each, median; keep the fastest as the world's shape in
+ * the context.  Returns it, its time, and (nullable) the number of shapes
+ * timed. */
 int vp_host_autotune_remesh(vp_ctx *ctx, const vp_world_desc *world,
                             const uint8_t *d_voxels, const int16_t *d_uniform,
                             const vp_shadow_desc *shadow, const uint32_t *d_chunk_ids,
                             uint32_t n, uint32_t flags, const vp_splat_out *out, int reps,
-                            void *stream, int *best_block, int *best_wg_per_cu,
-                            float *best_ms);
+                            void *stream, vp_remesh_shape *best, float *best_ms,
+                            int *shapes_timed);
 
 /* ---- on-disk RLE world -------------------------------------------------------
  * The file of command_export (src/deadcode.c:320-346), read back as

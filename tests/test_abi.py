@@ -46,6 +46,7 @@ def test_null_arguments_are_errors_not_crashes():
     assert lib.vp_gpu_splat(None, None, None, 0, 0, None, None, None, None) == abi.VP_ERR_ARG
     assert lib.vp_gpu_lod_mip(None, 6, 1, None, None, 1, None) == abi.VP_ERR_ARG
     assert lib.vp_ctx_create(0, None) == abi.VP_ERR_ARG
+    assert lib.vp_ctx_set_splat_tiles(None, 1, 0) == abi.VP_ERR_ARG
     assert lib.vp_strerror(abi.VP_ERR_CAPACITY) == b"output capacity exceeded"
     # host-side shadow updates run without a GPU
     w = abi.WorldDesc.make(2, (1, 1, 1))

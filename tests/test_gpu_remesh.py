@@ -49,21 +49,68 @@ def test_random_worlds_batched(gpu, rb, mb, density, seed):
         _check(w, vox, [c], smap, gpu_remesh(gpu, w, vox, [c], smap))
 
 
-@pytest.mark.parametrize("block,wg", [(256, 2), (1024, 1), (512, 2)])
-def test_launch_shapes(gpu, block, wg):
-    """Every workgroup size the autotuner may pick gives the same bytes
-    (the last case restores the default shape)."""
+@pytest.mark.parametrize("block,wg,u", [(256, 2, 4), (1024, 1, 4), (512, 2, 2), (256, 1, 2),
+                                        (1024, 1, 2), (512, 2, 4)])
+def test_launch_shapes(gpu, block, wg, u):
+    """Every launch shape the autotuner may pick gives the same bytes: set as
+    the world's own shape, then as the default (both restored after)."""
     cfg = W.small_world(6, (2, 1, 2), 0x5EED0077, hmax=100)
     w = cfg.desc()
     vox = O.gen_world(w, cfg.gen())
     smap = O.shadow_build(w, vox)
     ids = np.arange(w.chunks, dtype=np.uint32)
-    gpu.set_launch(block, wg)
+    default = gpu.get_remesh_shape()
+    w_other = abi.WorldDesc.make(6, (1, 1, 2))
+    gpu.set_remesh_shape(w, block, wg, u)
     try:
-        got = gpu_remesh(gpu, w, vox, ids, smap)
+        assert gpu.get_remesh_shape(w) == {"block": block, "wg_per_cu": wg,
+                                           "records_in_flight": u}
+        assert gpu.get_remesh_shape(w_other) == default     # other worlds keep the default
+        _check(w, vox, ids, smap, gpu_remesh(gpu, w, vox, ids, smap))
     finally:
-        gpu.set_launch(512, 2)
-    _check(w, vox, ids, smap, got)
+        gpu.set_remesh_shape(w, 512, 2, 4)
+    gpu.set_remesh_shape(None, block, wg, u)
+    try:
+        vox2 = O.gen_world(w_other, W.small_world(6, (1, 1, 2), 0x5EED0078, hmax=50).gen())
+        smap2 = O.shadow_build(w_other, vox2)
+        ids2 = np.arange(w_other.chunks, dtype=np.uint32)
+        _check(w_other, vox2, ids2, smap2, gpu_remesh(gpu, w_other, vox2, ids2, smap2))
+    finally:
+        gpu.set_remesh_shape(None, 512, 2, 4)
+
+
+def test_residency_and_autotuner(gpu):
+    """Resident workgroups per CU follow the kernels' LDS (about 67 KiB at
+    512 threads, 151 KiB at 1024); the autotuner times only resident shapes,
+    keeps the fastest as the world's own shape, and leaves the default and
+    other worlds alone; its pick remeshes to the oracle's bytes."""
+    import torch
+
+    cfg = W.small_world(6, (2, 1, 2), 0x5EED0079, hmax=100)
+    w = cfg.desc()
+    assert gpu.remesh_resident(w, 512) == 2 and gpu.remesh_resident(w, 512, 2) == 2
+    assert gpu.remesh_resident(w, 1024) == 1
+    res = {(b, u): gpu.remesh_resident(w, b, u) for b in (256, 512, 1024) for u in (2, 4)}
+    assert all(1 <= r for r in res.values())
+    vox = O.gen_world(w, cfg.gen())
+    smap = O.shadow_build(w, vox)
+    ids = np.arange(w.chunks, dtype=np.uint32)
+    default = gpu.get_remesh_shape()
+    vd, sd, idd = dev(vox), dev(smap), dev(ids)
+    _, offs, counts = gpu.cull_mesh(w, vd, idd, shadow_map=sd, count_only=True)
+    cap = int(offs[-1].item()) // 4
+    splats = torch.empty((cap, 4), dtype=torch.int16, device="cuda")
+    t = gpu.autotune_remesh(w, vd, idd, sd, splats, cap, offs, counts, reps=1)
+    assert t["shapes_timed"] == sum(min(r, 4) for r in res.values())
+    assert t["wg_per_cu"] <= res[(t["block"], t["records_in_flight"])]
+    best = {k: t[k] for k in ("block", "wg_per_cu", "records_in_flight")}
+    assert gpu.get_remesh_shape(w) == best
+    assert gpu.get_remesh_shape() == default
+    assert gpu.get_remesh_shape(abi.WorldDesc.make(6, (1, 1, 1))) == default
+    try:
+        _check(w, vox, ids, smap, gpu_remesh(gpu, w, vox, ids, smap))
+    finally:
+        gpu.set_remesh_shape(w, 512, 2, 4)
 
 
 @pytest.mark.parametrize("rb", [3, 6])

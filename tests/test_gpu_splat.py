@@ -174,3 +174,25 @@ def test_sprite_footprint_closed_form(gpu, blocks):
         pos = [int(round(c - 128)) for c in centre]
         ids, _ = _compare(gpu, np.array([[*pos, 9]], np.int16), cam, lod_log2=8, blocks=blocks)
         assert (ids != 0xFFFFFFFF).any() == any_hit
+
+
+@pytest.mark.parametrize("mode", ["tiles", "global", "fallback"])
+def test_tile_and_global_paths_match_oracle(gpu, terrain, mode):
+    """The block-culled frame drawn on screen tiles, by the global
+    per-fragment path (tiles off, the default), and by the global path taken as the
+    fallback of a tiled frame whose entries exceed the capacity (one entry):
+    all three give the oracle's ids.  The far view puts the whole world into a
+    few tiles, so each tile is split into several raster items merged with
+    atomics; the view among the hills has sprites too large for 2x2 tiles."""
+    w, recs, lod, l0, top = terrain
+    gpu.set_splat_tiles(mode != "global", 1 if mode == "fallback" else 0)
+    try:
+        far = W.make_camera((128.0, 160.0, 400.0), 0.0, 30.0, 64, 48)    # four tiles
+        ids, _ = _compare(gpu, l0, far, blocks=True)
+        assert (ids != 0xFFFFFFFF).mean() > 0.05
+        near = W.make_camera((128.0, 60.0, 128.0), 30.0, -5.0, 320, 180)
+        _compare(gpu, l0, near, blocks=True)
+        loc, yaw, pitch = W.splat_poses(top, n=1, seed=91, lo=40.0, hi=216.0)[0]
+        _compare(gpu, recs, W.make_camera(loc, yaw, pitch, 300, 170), lod=lod, blocks=True)
+    finally:
+        gpu.set_splat_tiles(False, 0)

@@ -113,6 +113,13 @@ class GPU:
     def num_cus(self) -> int:
         return self.lib.vp_ctx_num_cus(self.ctx)
 
+    def set_splat_tiles(self, enable: bool = True, max_entries: int = 0) -> None:
+        """Screen-tile resolve of block-culled frames on or off (off by
+        default), and its entry capacity (0: the default); the frames are
+        identical either way."""
+        _check(self.lib.vp_ctx_set_splat_tiles(self.ctx, 1 if enable else 0, max_entries),
+               "vp_ctx_set_splat_tiles")
+
     # ---- remesh --------------------------------------------------------------
     def cull_mesh(self, world: abi.WorldDesc, voxels, ids, shadow_map=None, uniform=None,
                   lod: bool = True, v2: bool = False, capacity: int | None = None,
@@ -741,6 +748,37 @@ def _gpu_get_launch(self: GPU):
     return b.value, w.value
 
 
+def _shape_dict(s: abi.RemeshShape) -> dict:
+    return {"block": s.block, "wg_per_cu": s.wg_per_cu, "records_in_flight": s.records_in_flight}
+
+
+def _gpu_set_remesh_shape(self: GPU, world: abi.WorldDesc | None, block: int, wg_per_cu: int,
+                          records_in_flight: int = 4) -> None:
+    """The remesh launch shape for `world` (None: the default for worlds
+    without their own)."""
+    s = abi.RemeshShape(block, wg_per_cu, records_in_flight)
+    _check(self.lib.vp_ctx_set_remesh_shape(self.ctx, None if world is None else C.byref(world),
+                                            C.byref(s)), "vp_ctx_set_remesh_shape")
+
+
+def _gpu_get_remesh_shape(self: GPU, world: abi.WorldDesc | None = None) -> dict:
+    s = abi.RemeshShape()
+    _check(self.lib.vp_ctx_get_remesh_shape(self.ctx, None if world is None else C.byref(world),
+                                            C.byref(s)), "vp_ctx_get_remesh_shape")
+    return _shape_dict(s)
+
+
+def _gpu_remesh_resident(self: GPU, world: abi.WorldDesc, block: int,
+                         records_in_flight: int = 4, v2: bool = False) -> int:
+    """Workgroups of that remesh kernel that fit on one CU at once."""
+    s = abi.RemeshShape(block, 1, records_in_flight)
+    n = C.c_int()
+    _check(self.lib.vp_ctx_remesh_resident(self.ctx, C.byref(world),
+                                           abi.VP_CM_RECORD_V2 if v2 else 0, C.byref(s),
+                                           C.byref(n)), "vp_ctx_remesh_resident")
+    return n.value
+
+
 def _gpu_remesh_per_chunk(self: GPU, world: abi.WorldDesc, voxels, ids, shadow_map, splats,
                           rec_offsets, rec_caps, offsets2, counts, uniform=None, lod=True,
                           stream=None):
@@ -771,13 +809,15 @@ def _gpu_autotune(self: GPU, world: abi.WorldDesc, voxels, ids, shadow_map, spla
     so = abi.SplatOut()
     so.d_splats, so.capacity_records = splats.data_ptr(), capacity
     so.d_chunk_offsets, so.d_level_counts = offsets.data_ptr(), counts.data_ptr()
-    b, w, ms = C.c_int(), C.c_int(), C.c_float()
+    best, ms, timed = abi.RemeshShape(), C.c_float(), C.c_int()
     _check(self.lib.vp_host_autotune_remesh(
         self.ctx, C.byref(world), C.c_void_p(voxels.data_ptr()), C.c_void_p(_ptr(uniform)),
         C.byref(sh), C.c_void_p(ids.data_ptr()), int(ids.numel()),
         abi.VP_CM_LOD if lod else 0, C.byref(so), reps, C.c_void_p(self._stream(stream)),
-        C.byref(b), C.byref(w), C.byref(ms)), "vp_host_autotune_remesh")
-    return b.value, w.value, ms.value
+        C.byref(best), C.byref(ms), C.byref(timed)), "vp_host_autotune_remesh")
+    out = _shape_dict(best)
+    out.update(ms=ms.value, shapes_timed=timed.value)
+    return out
 
 
 def _gpu_world_save(self: GPU, path, world: abi.WorldDesc, voxels, shadow_map=None,
@@ -814,6 +854,9 @@ GPU.world_save = _gpu_world_save
 GPU.world_load = _gpu_world_load
 GPU.reserve = _gpu_reserve
 GPU.set_launch = _gpu_set_launch
+GPU.set_remesh_shape = _gpu_set_remesh_shape
+GPU.get_remesh_shape = _gpu_get_remesh_shape
+GPU.remesh_resident = _gpu_remesh_resident
 GPU.get_launch = _gpu_get_launch
 GPU.remesh_per_chunk = _gpu_remesh_per_chunk
 GPU.autotune_remesh = _gpu_autotune

@@ -57,6 +57,10 @@ def shadow_padded_len(size_x: int, size_z: int) -> int:
     return size_x * (size_z + 17) + 64
 
 
+class RemeshShape(C.Structure):
+    _fields_ = [("block", C.c_int), ("wg_per_cu", C.c_int), ("records_in_flight", C.c_int)]
+
+
 class ShadowDesc(C.Structure):
     _fields_ = [("d_map", C.c_void_p), ("size_x", C.c_uint32), ("size_z", C.c_uint32)]
 
@@ -109,6 +113,7 @@ PROTOTYPES = {
     "vp_strerror": (C.c_char_p, [C.c_int]),
     "vp_ctx_status": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(C.c_uint32)]),
     "vp_ctx_num_cus": (C.c_int, [C.c_void_p]),
+    "vp_ctx_set_splat_tiles": (C.c_int, [C.c_void_p, C.c_int, C.c_uint64]),
     "vp_gpu_cull_mesh": (C.c_int, [C.c_void_p, C.POINTER(WorldDesc), C.c_void_p, C.c_void_p,
                                    C.POINTER(ShadowDesc), C.c_void_p, C.c_uint32, C.c_void_p,
                                    C.c_uint32, C.POINTER(SplatOut), C.c_void_p]),
@@ -205,6 +210,12 @@ PROTOTYPES.update({
     "vp_ctx_reserve": (C.c_int, [C.c_void_p, C.c_uint64, C.c_uint64]),
     "vp_ctx_set_launch": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
     "vp_ctx_get_launch": (C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    "vp_ctx_set_remesh_shape": (C.c_int, [C.c_void_p, C.POINTER(WorldDesc),
+                                          C.POINTER(RemeshShape)]),
+    "vp_ctx_get_remesh_shape": (C.c_int, [C.c_void_p, C.POINTER(WorldDesc),
+                                          C.POINTER(RemeshShape)]),
+    "vp_ctx_remesh_resident": (C.c_int, [C.c_void_p, C.POINTER(WorldDesc), C.c_uint32,
+                                         C.POINTER(RemeshShape), C.POINTER(C.c_int)]),
     "vp_host_remesh_per_chunk": (C.c_int, [C.c_void_p, C.POINTER(WorldDesc), C.c_void_p,
                                            C.c_void_p, C.POINTER(ShadowDesc), C.c_void_p,
                                            C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p,
@@ -223,8 +234,8 @@ PROTOTYPES.update({
     "vp_host_autotune_remesh": (C.c_int, [C.c_void_p, C.POINTER(WorldDesc), C.c_void_p,
                                           C.c_void_p, C.POINTER(ShadowDesc), C.c_void_p,
                                           C.c_uint32, C.c_uint32, C.POINTER(SplatOut), C.c_int,
-                                          C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int),
-                                          C.POINTER(C.c_float)]),
+                                          C.c_void_p, C.POINTER(RemeshShape),
+                                          C.POINTER(C.c_float), C.POINTER(C.c_int)]),
     "vp_host_frustum_planes": (None, [C.POINTER(C.c_float), C.POINTER(C.c_float)]),
     "vp_host_octree_cull": (C.c_int, [C.POINTER(WorldDesc), C.POINTER(C.c_float),
                                       C.POINTER(C.c_float), C.c_void_p * 5, C.c_void_p,

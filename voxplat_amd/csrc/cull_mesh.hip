@@ -20,9 +20,11 @@
 //  4. counts per row -> workgroup scan -> per-chunk total; the chunk's
 //     output offset comes from a decoupled look-back over the work-queue
 //     ticket order, so the concatenated output is in list order.
-//  5. emission: one wave per row, lane = x; the row word is the wave's ballot,
-//     so mbcnt gives each lane its slot and the stores are contiguous.
+//  5. emission: every z-slice of every level is a contiguous range of the
+//     output; waves take slices from a work counter in LDS and write them
+//     without workgroup barriers (emit_slices).
 #include "vp_internal.h"
+#include "../../include/vp_host.h"
 
 namespace vp {
 
@@ -70,41 +72,43 @@ struct Geo {
     }
 };
 
-// LDS layout in 64-bit words.  Once visibility is built the occupancy rows
-// are dead (unless v2 face flags still read them) and their space holds the
-// emission scratch: slice offsets, the staged slice pair, the row offsets of
-// the rows being expanded and the list of their cells.
+// LDS layout in 64-bit words.  The level-0 visibility rows V0 are built over
+// the occupancy rows S in place (every S row is read before any V0 row is
+// written), unless v2 face flags still need S afterwards.  Past the LOD rows
+// come the slice offsets, the emission work counter and one area per wave:
+// the wave's list of cells to emit and a window of one shadow-map row.
 template <int B, bool V2>
-struct Lay {
+struct Rows {
     using G = Geo<B>;
     static constexpr int R = G::R, N0 = G::N0;
     static constexpr int S = 0;
-    static constexpr int HX = S + R * R;
+    static constexpr int V0 = V2 ? R * R : 0;
+    static constexpr int HX = V0 + N0 * N0;
     static constexpr int HY = HX + R;
     static constexpr int HZ = HY + R;
-    static constexpr int V0 = HZ + R;
-    static constexpr int XB = V0 + N0 * N0;
+    static constexpr int XB = HZ + R;
     static constexpr int P1 = XB + R;               // P1..P4 contiguous, row order
     static constexpr int PEND = P1 + (G::ROWS - G::RB1);
     // z-slices of every level, level-major: slice base of level l
     static constexpr int SLB0 = 0, SLB1 = G::n(0), SLB2 = SLB1 + G::n(1), SLB3 = SLB2 + G::n(2),
                          SLB4 = SLB3 + G::n(3), NSLICES = SLB4 + G::n(4);
-    // staged slice pair: two z-slices, +x column and +y row bytes, 3 shadow rows
-    static constexpr int SW = 2 * R + 8;              // shadow entries per staged row
-    static constexpr int ST_BYTES = 2 * R * R + 4 * R + 3 * SW * 2;
-    static constexpr int maxi(int a, int b) { return a > b ? a : b; }
-    static constexpr int al16(int v) { return (v + 15) & ~15; }
-    static constexpr int PR_ROWS = maxi(2 * N0 + G::n(1), G::n(2) * G::n(2));
-    static constexpr int LIST_N = maxi(2 * N0 * N0 + G::n(1) * G::n(1),
-                                       G::n(2) * G::n(2) * G::n(2));
-    static constexpr int O_STG = 0;
-    static constexpr int O_SOFF = al16(ST_BYTES);
-    static constexpr int O_ROWP = O_SOFF + al16((NSLICES + 1) * 4);
-    static constexpr int O_LIST = O_ROWP + al16(PR_ROWS * 2);
-    static constexpr int SCR_BYTES = O_LIST + al16(LIST_N * 2);
-    static constexpr bool ALIAS = !V2 && SCR_BYTES <= R * R * 8;
-    static constexpr int SCR = ALIAS ? S : PEND;
-    static constexpr int TOTAL = ALIAS ? PEND : PEND + SCR_BYTES / 8;
+};
+
+template <int B, int BS, bool V2>
+struct Lay : Rows<B, V2> {
+    using P = Rows<B, V2>;
+    static constexpr int NW = BS / 64;
+    // u32 words: slice offsets [0, NSLICES], then the work counter
+    static constexpr int SOFF = P::PEND;
+    static constexpr int ITEM_U32 = P::NSLICES + 1;
+    static constexpr int SOFF_WORDS = (P::NSLICES + 2 + 1) / 2;
+    // per wave: LCAP list entries (u32: cell code, colour) and SWIN shadow entries
+    static constexpr int mini(int a, int b) { return a < b ? a : b; }
+    static constexpr int LCAP = mini(512, P::N0 * P::N0);
+    static constexpr int SWIN = 2 * P::R + 2;
+    static constexpr int WAVE_WORDS = (4 * LCAP + 2 * SWIN + 7) / 8;
+    static constexpr int WAVES = SOFF + SOFF_WORDS;
+    static constexpr int TOTAL = WAVES + NW * WAVE_WORDS;
 };
 
 __device__ __forceinline__ uint64_t compress_pairs(uint64_t r)
@@ -265,7 +269,7 @@ __device__ __forceinline__ uint8_t l0_colour(const ChunkRefs &c, int x, int y, i
 template <int B, bool V2>
 __device__ __forceinline__ uint32_t l0_bit(const uint64_t *lds, int x, int y, int z)
 {
-    using L = Lay<B, V2>;
+    using L = Rows<B, V2>;
     constexpr int R = 1 << B, N0 = R + 1;
     if (x < R) return (uint32_t)(lds[L::V0 + z * N0 + y] >> x) & 1u;
     if (y < R && z < R) return (uint32_t)(lds[L::XB + z] >> y) & 1u;
@@ -278,7 +282,7 @@ __device__ __forceinline__ uint32_t l0_bit(const uint64_t *lds, int x, int y, in
 template <int B, bool V2>
 __device__ __forceinline__ void descend(const uint64_t *lds, int l, int &x, int &y, int &z)
 {
-    using L = Lay<B, V2>;
+    using L = Rows<B, V2>;
     using G = Geo<B>;
     for (int lv = l; lv >= 1; --lv) {
         const int np = G::n(lv - 1);
@@ -307,12 +311,60 @@ __device__ __forceinline__ void descend(const uint64_t *lds, int l, int &x, int 
     }
 }
 
+// The descent of descend() without branches: at each step the four child
+// rows (z 2P+1, 2P; y 2Q+1, 2Q) are read together and the greatest (z,y,x)
+// non-zero child is picked with bit operations.
+template <int B, bool V2>
+__device__ __forceinline__ void descend_bf(const uint64_t *lds, int l, int &x, int &y, int &z)
+{
+    using L = Rows<B, V2>;
+    using G = Geo<B>;
+    constexpr int R = G::R, N0 = G::N0;
+    for (int lv = l; lv >= 1; --lv) {
+        const int np = G::n(lv - 1);
+        const int X0 = 2 * x, Y0 = 2 * y, Z0 = 2 * z;
+        uint32_t b[2][2][2];          // [dz][dy][dx]
+#pragma unroll
+        for (int dz = 0; dz < 2; dz++)
+#pragma unroll
+            for (int dy = 0; dy < 2; dy++) {
+                const int Z = Z0 + dz, Y = Y0 + dy;
+                uint64_t row = 0;
+                uint32_t bitR = 0;             // level 0: cell x = R of this row
+                if (Z < np && Y < np) {
+                    if (lv - 1 == 0) {
+                        row = lds[L::V0 + Z * N0 + Y];
+                        if (Z < R && Y < R) bitR = (uint32_t)(lds[L::XB + Z] >> Y) & 1u;
+                    } else {
+                        row = lds[L::P1 + (G::row_base(lv - 1) - G::RB1) + Z * np + Y];
+                    }
+                }
+                // bit X of the child row, X < np; at level 0, X = R is bitR
+                auto bit = [&](int X) -> uint32_t {
+                    if (X >= np) return 0u;
+                    if (lv - 1 == 0 && X == R) return bitR;
+                    return (uint32_t)(row >> X) & 1u;
+                };
+                b[dz][dy][1] = bit(X0 + 1);
+                b[dz][dy][0] = bit(X0);
+            }
+        const uint32_t a11 = b[1][1][1] | b[1][1][0], a10 = b[1][0][1] | b[1][0][0];
+        const uint32_t a01 = b[0][1][1] | b[0][1][0];
+        const uint32_t dz = a11 | a10;
+        const uint32_t dy = dz ? a11 : a01;
+        const uint32_t dx = b[dz][dy][1];
+        x = X0 + (int)dx;
+        y = Y0 + (int)dy;
+        z = Z0 + (int)dz;
+    }
+}
+
 // Face-exposure flags of level-0 cell q (v2 records): bit 2a = +a face open,
 // bit 2a+1 = -a face open, as the checks of mesher.c:427-445 found them.
 template <int B, bool V2>
 __device__ __forceinline__ uint32_t face_flags(const uint64_t *lds, int x, int y, int z)
 {
-    using L = Lay<B, V2>;
+    using L = Rows<B, V2>;
     constexpr int R = 1 << B;
     if (x == R) return 2u;          // seen through the -x face from this chunk
     if (y == R) return 8u;
@@ -381,162 +433,286 @@ __device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t *wsum, 
     return wsum[w] + x - v;
 }
 
-// ---- emission helpers --------------------------------------------------------
-// Registers carrying one slice pair from global memory to the LDS staging.
-template <int B, int BS>
-struct PairRegs {
-    static constexpr int R = 1 << B;
-    static constexpr int NSL = (2 * R * R + 15) / 16;   // 16-byte pieces of two slices
-    static constexpr int NXY = 4 * R;                   // +x column and +y row bytes
-    static constexpr int NSH = 3 * (2 * R + 8);         // shadow entries
-    static constexpr int KSL = (NSL + BS - 1) / BS;
-    static constexpr int KXY = (NXY + BS - 1) / BS;
-    static constexpr int KSH = (NSH + BS - 1) / BS;
-    uint4 sl[KSL];
-    uint32_t xy[KXY];
-    uint32_t sh[KSH];
-};
-
-__device__ __forceinline__ uint4 splat4(uint32_t v)
+// ---- emission ----------------------------------------------------------------
+// Exclusive prefix sum over the lanes of a wave, and the wave's total.
+__device__ __forceinline__ uint32_t wave_excl_scan(uint32_t v, uint32_t &total)
 {
-    const uint32_t w = (v & 0xFFu) * 0x01010101u;
-    return make_uint4(w, w, w, w);
+    const uint32_t ln = lane_id();
+    uint32_t x = v;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const uint32_t y = (uint32_t)__shfl_up((int)x, d, 64);
+        if (ln >= (uint32_t)d) x += y;
+    }
+    total = uni((uint32_t)__shfl((int)x, 63, 64));
+    return x - v;
 }
 
-// Issue the loads of pair (z0, z0+1); slice z = R is the +z neighbour's z=0 plane.
-template <int B, int BS>
-__device__ __forceinline__ void pair_load(const CullArgs &a, const ChunkRefs &c, int z0,
-                                          PairRegs<B, BS> &pr)
+__device__ __forceinline__ uint32_t wave_min(uint32_t v)
 {
-    constexpr int R = 1 << B, SL = R * R;
-    using P = PairRegs<B, BS>;
 #pragma unroll
-    for (int j = 0; j < P::KSL; j++) {
-        const int i = (int)threadIdx.x + j * BS;
-        uint4 v = make_uint4(0, 0, 0, 0);
-        if (i < P::NSL) {
-            const int byte = i * 16, sidx = byte / SL, off = byte % SL, z = z0 + sidx;
-            if (SL >= 16) {
-                if (z < R) {
-                    v = c.own_u >= 0 ? splat4((uint32_t)c.own_u)
-                                     : *reinterpret_cast<const uint4 *>(c.own + (size_t)z * SL + off);
-                } else if (z == R) {
-                    v = c.nb_u[2] >= 0 ? splat4((uint32_t)c.nb_u[2])
-                                       : *reinterpret_cast<const uint4 *>(c.nb[2] + off);
-                }
-            } else {   // slices smaller than 16 bytes (R = 2): byte by byte
-                uint32_t wv[4] = {0, 0, 0, 0};
-                for (int q = 0; q < 16; q++) {
-                    const int bb = byte + q, zz = z0 + bb / SL, oo = bb % SL;
-                    uint32_t x = 0;
-                    if (zz < R) x = c.own_u >= 0 ? (uint32_t)c.own_u : c.own[(size_t)zz * SL + oo];
-                    else if (zz == R) x = c.nb_u[2] >= 0 ? (uint32_t)c.nb_u[2] : c.nb[2][oo];
-                    wv[q >> 2] |= (x & 0xFFu) << (8 * (q & 3));
-                }
-                v = make_uint4(wv[0], wv[1], wv[2], wv[3]);
-            }
-        }
-        pr.sl[j] = v;
-    }
-#pragma unroll
-    for (int j = 0; j < P::KXY; j++) {
-        const int i = (int)threadIdx.x + j * BS;
-        uint32_t v = 0;
-        if (i < P::NXY) {
-            const int kind = i / (2 * R), sidx = (i % (2 * R)) / R, e = i % R, z = z0 + sidx;
-            if (z < R) {
-                if (kind == 0)   // voxel (0, e, z) of the +x neighbour
-                    v = c.nb_u[0] >= 0 ? (uint32_t)c.nb_u[0] : c.nb[0][(size_t)(z * R + e) * R];
-                else             // voxel (e, 0, z) of the +y neighbour
-                    v = c.nb_u[1] >= 0 ? (uint32_t)c.nb_u[1] : c.nb[1][(size_t)(z * R) * R + e];
-            }
-        }
-        pr.xy[j] = v;
-    }
-#pragma unroll
-    for (int j = 0; j < P::KSH; j++) {
-        const int i = (int)threadIdx.x + j * BS;
-        uint32_t v = 0;
-        if (i < P::NSH && a.map) {
-            const int SWn = 2 * R + 8, row = i / SWn, e = i % SWn;
-            v = a.map[(size_t)a.sx * (c.org[2] + z0 + row) + (c.org[0] + c.org[1]) + e];
-        }
-        pr.sh[j] = v;
-    }
+    for (int d = 32; d >= 1; d >>= 1) v = min(v, (uint32_t)__shfl_xor((int)v, d, 64));
+    return uni(v);
 }
 
-template <int B, int BS>
-__device__ __forceinline__ void pair_store(const PairRegs<B, BS> &pr, uint8_t *stg)
+// The cell list and the shadow window of a wave are written and read by
+// different lanes of that wave: keep the accesses in program order.
+__device__ __forceinline__ void wave_sync()
 {
-    constexpr int R = 1 << B;
-    using P = PairRegs<B, BS>;
-    constexpr int XC = 2 * R * R, SH = XC + 4 * R;
-#pragma unroll
-    for (int j = 0; j < P::KSL; j++) {
-        const int i = (int)threadIdx.x + j * BS;
-        if (i < P::NSL) *reinterpret_cast<uint4 *>(stg + i * 16) = pr.sl[j];
-    }
-#pragma unroll
-    for (int j = 0; j < P::KXY; j++) {
-        const int i = (int)threadIdx.x + j * BS;
-        if (i < P::NXY) stg[XC + i] = (uint8_t)pr.xy[j];
-    }
-#pragma unroll
-    for (int j = 0; j < P::KSH; j++) {
-        const int i = (int)threadIdx.x + j * BS;
-        if (i < P::NSH) reinterpret_cast<uint16_t *>(stg + SH)[i] = (uint16_t)pr.sh[j];
-    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// Colour of level-0 cell (x, y, z) of the staged pair z0, z0+1.
+// Colour of level-0 cell (x, y, z), x, y, z in [0, R]: one select of the
+// source and at most one byte load.
 template <int B>
-__device__ __forceinline__ uint32_t staged_colour(const uint8_t *stg, int x, int y, int z, int z0)
+__device__ __forceinline__ uint32_t cell_colour(const ChunkRefs &c, int x, int y, int z)
 {
     constexpr int R = 1 << B;
-    const int sidx = z - z0;
-    if (x == R) return stg[2 * R * R + sidx * R + y];
-    if (y == R) return stg[2 * R * R + 2 * R + sidx * R + x];
-    return stg[sidx * R * R + y * R + x];
+    const uint8_t *p = c.own;
+    int u = c.own_u;
+    size_t i = (size_t)((z * R + y) * R + x);
+    if (x == R) {
+        p = c.nb[0]; u = c.nb_u[0]; i = (size_t)((z * R + y) * R);
+    } else if (y == R) {
+        p = c.nb[1]; u = c.nb_u[1]; i = (size_t)(z * R * R + x);
+    } else if (z == R) {
+        p = c.nb[2]; u = c.nb_u[2]; i = (size_t)(y * R + x);
+    }
+    return u >= 0 ? (uint32_t)u : (uint32_t)p[i];
 }
 
-// shadow_sample on a staged row: entry e is map[sx*(oz+z0+row) + ox+oy + e]
-template <int B>
-__device__ __forceinline__ uint32_t staged_shadow(const uint8_t *stg, int row, int e, uint32_t wy)
+// Even bits of r (bit 2k -> bit k), 32 of them.
+__device__ __forceinline__ uint64_t even_bits(uint64_t r)
+{
+    uint64_t x = r & 0x5555555555555555ull;
+    x = (x | (x >> 1)) & 0x3333333333333333ull;
+    x = (x | (x >> 2)) & 0x0F0F0F0F0F0F0F0Full;
+    x = (x | (x >> 4)) & 0x00FF00FF00FF00FFull;
+    x = (x | (x >> 8)) & 0x0000FFFF0000FFFFull;
+    x = (x | (x >> 16)) & 0x00000000FFFFFFFFull;
+    return x;
+}
+
+// Level-0 row (y, z) as bits x = 0..R: the V0 row and, at bit R, the cell
+// (R, y, z) of the +x neighbour (zero outside the mask's (R+1)^3 cells).
+template <int B, bool V2>
+__device__ __forceinline__ void l0_row65(const uint64_t *lds, int y, int z, uint64_t &lo64,
+                                        uint32_t &bitR)
+{
+    using L = Rows<B, V2>;
+    constexpr int R = 1 << B, N0 = R + 1;
+    lo64 = 0;
+    bitR = 0;
+    if (y > R || z > R) return;
+    lo64 = lds[L::V0 + z * N0 + y];
+    if (y < R && z < R) bitR = (uint32_t)(lds[L::XB + z] >> y) & 1u;
+}
+
+// For level-1 row (Y, slice s): the level-0 child each cell X descends to --
+// the greatest (z, y, x) non-zero child, the last write of the downsample
+// scan (mesher.c:460-493) -- as three bit planes over X: dz (child in z =
+// 2s+1), dy (child in y = 2Y+1), dx (child at x = 2X+1).
+template <int B, bool V2>
+__device__ __forceinline__ void l1_child_planes(const uint64_t *lds, int s, int Y,
+                                                uint64_t &pdz, uint64_t &pdy, uint64_t &pdx)
 {
     constexpr int R = 1 << B;
-    const uint16_t *sh = reinterpret_cast<const uint16_t *>(stg + 2 * R * R + 4 * R);
-    const uint16_t *rw = sh + row * (2 * R + 8);
-    return !(rw[e] <= wy && rw[e + 1] <= wy);
-}
-
-// Exclusive scan of the counts of rows [0, nrows) into rowp (u16, relative
-// to row 0); each thread sums a contiguous block of rows.  Returns the total.
-template <int BS, typename F>
-__device__ __forceinline__ uint32_t scan_rows(int nrows, F count, uint16_t *rowp, uint32_t *wsum)
-{
-    const int K = (nrows + BS - 1) / BS;
-    const int r0 = (int)threadIdx.x * K;
-    uint32_t mine = 0;
-    for (int j = 0; j < K; j++)
-        if (r0 + j < nrows) mine += count(r0 + j);
-    uint32_t tot = 0;
-    uint32_t run = block_excl_scan<BS>(mine, wsum, tot);
-    for (int j = 0; j < K; j++)
-        if (r0 + j < nrows) {
-            rowp[r0 + j] = (uint16_t)run;
-            run += count(r0 + j);
+    uint64_t hi[2][2], lo[2][2], any[2][2];
+#pragma unroll
+    for (int dz = 0; dz < 2; dz++)
+#pragma unroll
+        for (int dy = 0; dy < 2; dy++) {
+            uint64_t r;
+            uint32_t br;
+            l0_row65<B, V2>(lds, 2 * Y + dy, 2 * s + dz, r, br);
+            // cell X: children x = 2X (even bit) and 2X + 1 (odd bit); the
+            // cell X = R/2 has only x = R, bit R of the 65-bit row
+            lo[dz][dy] = even_bits(r) | ((uint64_t)br << (R / 2));
+            hi[dz][dy] = even_bits(r >> 1);
+            any[dz][dy] = lo[dz][dy] | hi[dz][dy];
         }
-    __syncthreads();
-    return tot;
+    pdz = any[1][1] | any[1][0];
+    pdy = (pdz & any[1][1]) | (~pdz & any[0][1]);
+    pdx = (pdz & pdy & hi[1][1]) | (pdz & ~pdy & hi[1][0]) | (~pdz & pdy & hi[0][1]) |
+          (~pdz & ~pdy & hi[0][0]);
+}
+
+// Emission of one chunk whose slice offsets are in LDS.  Records are ordered
+// level, z, y, x, so each z-slice of each level is a contiguous range of the
+// output starting at its slice offset.  Waves take slices from the work
+// counter, with no workgroup barrier: lane y counts row y of the slice (plus
+// the cell x = R of level 0), a wave scan gives each row its place, each lane
+// writes its row's cells (y << 7 | x) to the wave's list, and then the wave
+// writes the list's records with consecutive lanes on consecutive records,
+// U records per lane in flight.  A slice with more than LCAP cells goes in
+// batches of whole rows.  Colours are byte loads from the chunk (or from the
+// neighbour planes for cells at coordinate R); the shadow bits read a window
+// of the one map row the slice samples, staged in the wave's LDS area.
+template <int B, int BS, bool V2, int U>
+__device__ void emit_slices(const CullArgs &a, const ChunkRefs &c, uint64_t *lds, uint64_t *out)
+{
+    using L = Lay<B, BS, V2>;
+    using G = Geo<B>;
+    constexpr int R = G::R, N0 = G::N0;
+    constexpr int LCAP = L::LCAP, SWIN = L::SWIN;
+    const uint32_t ln = lane_id(), w = threadIdx.x >> 6;
+    uint32_t *soff = reinterpret_cast<uint32_t *>(lds + L::SOFF);
+    uint32_t *list = reinterpret_cast<uint32_t *>(lds + L::WAVES + (int)w * L::WAVE_WORDS);
+    uint16_t *win = reinterpret_cast<uint16_t *>(list + LCAP);
+    const int nitems = a.lod ? L::NSLICES : L::SLB1;
+#pragma unroll 1
+    for (;;) {
+        uint32_t it = 0;
+        if (ln == 0)
+            it = __hip_atomic_fetch_add(&soff[L::ITEM_U32], 1u, __ATOMIC_RELAXED,
+                                        __HIP_MEMORY_SCOPE_WORKGROUP);
+        it = uni((uint32_t)__shfl((int)it, 0, 64));
+        if ((int)it >= nitems) break;
+        const int l = (int)it < L::SLB1 ? 0 : (int)it < L::SLB2 ? 1 : (int)it < L::SLB3 ? 2
+                    : (int)it < L::SLB4 ? 3 : 4;
+        const int s = (int)it - (l == 0 ? L::SLB0 : l == 1 ? L::SLB1 : l == 2 ? L::SLB2
+                                 : l == 3 ? L::SLB3 : L::SLB4);
+        const int nl = G::n(l);
+        const uint32_t sbeg = soff[it], T = soff[it + 1] - sbeg;
+        if (T == 0) continue;
+        const uint32_t d = l ? (1u << l) : 0u;        // LOD samples at +(1 << l) (mesher.c:523-531)
+        wave_sync();                                   // the previous slice's window reads are done
+        if (a.map) {
+            const size_t wb = (size_t)a.sx * (c.org[2] + ((uint32_t)s << l) + d) +
+                              c.org[0] + c.org[1] + 2u * d;
+            for (int e = (int)ln; e < SWIN; e += 64) win[e] = a.map[wb + e];
+        }
+        const uint64_t *rows = l == 0 ? lds + L::V0 + s * N0
+                                      : lds + L::P1 + (G::row_base(l) - G::RB1) + s * nl;
+        uint64_t rb = 0;
+        uint32_t xb = 0;
+        if ((int)ln < nl) rb = rows[ln];
+        if (l == 0 && s < R && (int)ln < R) xb = (uint32_t)(lds[L::XB + s] >> ln) & 1u;
+        const uint32_t cnt = (uint32_t)__popcll(rb) + xb;
+        uint32_t tot64;
+        const uint32_t off = wave_excl_scan(cnt, tot64);
+        const uint64_t rbx = nl > 64 ? rows[64] : 0ull;    // row y = 64 of level 0 (R = 64)
+        uint64_t pdz = 0, pdy = 0, pdx = 0;     // level-1 child planes of lane's row
+        uint64_t *o = out + sbeg;
+        const uint32_t wz = c.org[2] + ((uint32_t)s << l);
+#pragma unroll 1
+        for (uint32_t b0 = 0; b0 < T;) {
+            const uint32_t lim = b0 + LCAP;
+            const bool pending = cnt && off >= b0;
+            uint32_t b1 = wave_min(pending && off + cnt > lim ? off : 0xFFFFFFFFu);
+            if (b1 == 0xFFFFFFFFu) b1 = T <= lim ? T : tot64;
+            wave_sync();                               // the previous batch's list reads are done
+            if (pending && off < b1) {
+                uint32_t pos = off - b0;
+                uint64_t m = rb;
+                const uint32_t code = ln << 7;
+                if (l == 1) {
+                    // level 1: the descent to the level-0 child for the whole row
+                    // at once, as bit planes over X (the child's dz, dy, dx)
+                    l1_child_planes<B, V2>(lds, s, (int)ln, pdz, pdy, pdx);
+                    while (m) {
+                        const uint32_t x = (uint32_t)__builtin_ctzll(m);
+                        m &= m - 1ull;
+                        const uint32_t sel = (uint32_t)((pdz >> x) & 1ull) |
+                                             ((uint32_t)((pdy >> x) & 1ull) << 1) |
+                                             ((uint32_t)((pdx >> x) & 1ull) << 2);
+                        list[pos++] = code | x | (sel << 16);
+                    }
+                } else {
+                    while (m) {
+                        const uint32_t x = (uint32_t)__builtin_ctzll(m);
+                        m &= m - 1ull;
+                        list[pos++] = code | x;
+                    }
+                    if (xb) list[pos] = code | (uint32_t)R;
+                }
+            }
+            if (rbx && b1 == T && tot64 >= b0 && ((rbx >> ln) & 1ull))
+                list[tot64 - b0 + rank_below(rbx)] = (64u << 7) | ln;
+            wave_sync();
+            const uint32_t nb = b1 - b0;
+            uint64_t *ob = o + b0;
+            if (l == 0) {
+#pragma unroll 1
+                for (uint32_t j0 = 0; j0 < nb; j0 += 64 * U) {
+                    uint32_t col[U];
+                    uint64_t rec[U];
+#pragma unroll
+                    for (int u = 0; u < U; u++) {
+                        const uint32_t j = j0 + (uint32_t)u * 64u + ln;
+                        col[u] = 0;
+                        rec[u] = 0;
+                        if (j < nb) {
+                            const uint32_t e = list[j];
+                            const int x = (int)(e & 127u), y = (int)((e >> 7) & 127u);
+                            col[u] = cell_colour<B>(c, x, y, s);
+                            const uint32_t wy = c.org[1] + (uint32_t)y;
+                            uint32_t sh = 0;
+                            if (a.map) sh = !(win[x + y] <= wy && win[x + y + 1] <= wy);
+                            uint32_t fl = 0;
+                            if constexpr (V2) fl = face_flags<B, V2>(lds, x, y, s);
+                            rec[u] = pack_record<V2>(c.org[0] + (uint32_t)x, wy, wz, 0u, sh, fl);
+                        }
+                    }
+#pragma unroll
+                    for (int u = 0; u < U; u++) {
+                        const uint32_t j = j0 + (uint32_t)u * 64u + ln;
+                        if (j < nb) __builtin_nontemporal_store(rec[u] | ((uint64_t)(col[u] & 0xFFu) << 48), &ob[j]);
+                    }
+                }
+            } else {
+#pragma unroll 1
+                for (uint32_t j0 = 0; j0 < nb; j0 += 64 * U) {
+                    uint32_t col[U];
+                    uint64_t rec[U];
+#pragma unroll
+                    for (int u = 0; u < U; u++) {
+                        const uint32_t j = j0 + (uint32_t)u * 64u + ln;
+                        col[u] = 0;
+                        rec[u] = 0;
+                        if (j < nb) {
+                            const uint32_t e = list[j];
+                            const int X = (int)(e & 127u), Y = (int)((e >> 7) & 127u);
+                            int qx = X, qy = Y, qz = s;
+                            if (l == 1) {
+                                qz = 2 * s + (int)((e >> 16) & 1u);
+                                qy = 2 * Y + (int)((e >> 17) & 1u);
+                                qx = 2 * X + (int)((e >> 18) & 1u);
+                            } else {
+                                descend_bf<B, V2>(lds, l, qx, qy, qz);
+                            }
+                            col[u] = cell_colour<B>(c, qx, qy, qz);
+                            const uint32_t wy = c.org[1] + ((uint32_t)Y << l);
+                            const uint32_t ex = (uint32_t)(X + Y) << l;
+                            uint32_t sh = 0;
+                            if (a.map) sh = !(win[ex] <= wy + d && win[ex + 1] <= wy + d);
+                            uint32_t fl = 0;
+                            if constexpr (V2) fl = face_flags<B, V2>(lds, qx, qy, qz);
+                            rec[u] = pack_record<V2>(c.org[0] + ((uint32_t)X << l), wy, wz, 0u,
+                                                     sh, fl);
+                        }
+                    }
+#pragma unroll
+                    for (int u = 0; u < U; u++) {
+                        const uint32_t j = j0 + (uint32_t)u * 64u + ln;
+                        if (j < nb) __builtin_nontemporal_store(rec[u] | ((uint64_t)(col[u] & 0xFFu) << 48), &ob[j]);
+                    }
+                }
+            }
+            b0 = b1;
+        }
+    }
 }
 
 // Work for one chunk; all threads of the workgroup call it.
-template <int B, int BS, bool V2, bool MASK_OUT>
+template <int B, int BS, bool V2, bool MASK_OUT, int U>
 __device__ void cull_chunk(const CullArgs &a, uint32_t k, uint32_t n, uint64_t *lds,
                            uint32_t *wsum, uint64_t *s_base)
 {
     using G = Geo<B>;
-    using L = Lay<B, V2>;
+    using L = Lay<B, BS, V2>;
     constexpr int R = G::R, N0 = G::N0;
     constexpr int NW = BS / 64;
     constexpr uint64_t RM = G::ROWMASK;
@@ -613,35 +789,55 @@ __device__ void cull_chunk(const CullArgs &a, uint32_t k, uint32_t n, uint64_t *
         __syncthreads();
 
         // ---- 2. level-0 visibility (mesher.c:421-448 as row algebra) ----
-#pragma unroll 1
-        for (int base = 0; base < R * R; base += BS) {
-            const int r = base + (int)threadIdx.x;
-            const bool act = r < R * R;
-            uint32_t bx = 0;
-            if (act) {
-                const int z = r >> B, y = r & (R - 1);
-                const uint64_t s = lds[L::S + r];
-                const uint64_t hx = (lds[L::HX + z] >> y) & 1ull;
-                const uint64_t px = (s >> 1) | (hx << (R - 1));
-                uint64_t open = ~px | (~(s << 1) & ~1ull);
-                open |= ~((y + 1 < R) ? lds[L::S + r + 1] : lds[L::HY + z]);
-                if (y > 0) open |= ~lds[L::S + r - 1];
-                open |= ~((z + 1 < R) ? lds[L::S + r + R] : lds[L::HZ + y]);
-                if (z > 0) open |= ~lds[L::S + r - R];
-                lds[L::V0 + z * N0 + y] = s & open & RM;
-                bx = (uint32_t)(hx & ~(s >> (R - 1)) & 1ull);
+        // Each thread computes its rows into registers; V0 overwrites S only
+        // after the barrier, when every S row has been read.
+        {
+            constexpr int RPT = (R * R + BS - 1) / BS;
+            uint64_t vrow[RPT];
+#pragma unroll
+            for (int i = 0; i < RPT; i++) {
+                const int r = i * BS + (int)threadIdx.x;
+                const bool act = r < R * R;
+                uint32_t bx = 0;
+                uint64_t v = 0;
+                if (act) {
+                    const int z = r >> B, y = r & (R - 1);
+                    const uint64_t s = lds[L::S + r];
+                    const uint64_t hx = (lds[L::HX + z] >> y) & 1ull;
+                    const uint64_t px = (s >> 1) | (hx << (R - 1));
+                    uint64_t open = ~px | (~(s << 1) & ~1ull);
+                    open |= ~((y + 1 < R) ? lds[L::S + r + 1] : lds[L::HY + z]);
+                    if (y > 0) open |= ~lds[L::S + r - 1];
+                    open |= ~((z + 1 < R) ? lds[L::S + r + R] : lds[L::HZ + y]);
+                    if (z > 0) open |= ~lds[L::S + r - R];
+                    v = s & open & RM;
+                    bx = (uint32_t)(hx & ~(s >> (R - 1)) & 1ull);
+                }
+                vrow[i] = v;
+                const uint64_t bal = __ballot(bx);
+                const int ln = (int)lane_id();
+                if (act && (ln % R) == 0) lds[L::XB + (r >> B)] = (bal >> ln) & RM;
             }
-            const uint64_t bal = __ballot(bx);
-            const int ln = (int)lane_id();
-            if (act && (ln % R) == 0) lds[L::XB + (r >> B)] = (bal >> ln) & RM;
-        }
-        for (int i = threadIdx.x; i < R; i += BS) {
-            // cells (x, R, z): +y neighbour voxel exposed to this chunk's (x, R-1, z)
-            lds[L::V0 + i * N0 + R] = lds[L::HY + i] & ~lds[L::S + i * R + (R - 1)] & RM;
+            // cells (x, R, z): +y neighbour voxel exposed to this chunk's (x, R-1, z);
             // cells (x, y, R): +z neighbour voxel exposed to (x, y, R-1)
-            lds[L::V0 + R * N0 + i] = lds[L::HZ + i] & ~lds[L::S + (R - 1) * R + i] & RM;
+            uint64_t ey = 0, ez = 0;
+            const int t = (int)threadIdx.x;
+            if (t < R) {
+                ey = lds[L::HY + t] & ~lds[L::S + t * R + (R - 1)] & RM;
+                ez = lds[L::HZ + t] & ~lds[L::S + (R - 1) * R + t] & RM;
+            }
+            __syncthreads();
+#pragma unroll
+            for (int i = 0; i < RPT; i++) {
+                const int r = i * BS + (int)threadIdx.x;
+                if (r < R * R) lds[L::V0 + (r >> B) * N0 + (r & (R - 1))] = vrow[i];
+            }
+            if (t < R) {
+                lds[L::V0 + t * N0 + R] = ey;
+                lds[L::V0 + R * N0 + t] = ez;
+            }
+            if (t == 0) lds[L::V0 + R * N0 + R] = 0ull;
         }
-        if (threadIdx.x == 0) lds[L::V0 + R * N0 + R] = 0ull;
         __syncthreads();
 
         // ---- 3. LOD bit pyramid ----
@@ -695,8 +891,7 @@ __device__ void cull_chunk(const CullArgs &a, uint32_t k, uint32_t n, uint64_t *
         // ---- 4. per-slice counts -> scan -> slice offsets ----
         // Records are ordered by level, z, y, x, so every z-slice of every
         // level owns a contiguous range: one offset per slice is enough.
-        uint32_t *soff = reinterpret_cast<uint32_t *>(
-            reinterpret_cast<uint8_t *>(lds + L::SCR) + L::O_SOFF);
+        uint32_t *soff = reinterpret_cast<uint32_t *>(lds + L::SOFF);
         {
             const int w = (int)(threadIdx.x >> 6);
             const uint32_t ln = lane_id();
@@ -743,6 +938,7 @@ __device__ void cull_chunk(const CullArgs &a, uint32_t k, uint32_t n, uint64_t *
                     }
                 }
                 if (ln == 63) soff[L::NSLICES] = x;
+                if (ln == 0) soff[L::ITEM_U32] = 0u;   // emission work counter
             }
             __syncthreads();
         }
@@ -813,170 +1009,29 @@ __device__ void cull_chunk(const CullArgs &a, uint32_t k, uint32_t n, uint64_t *
     const uint64_t base = uni(*s_base) / 4;
     if (base + total > a.cap) return;
 
-    // ---- 5. emission ----
-    // The chunk is walked in z-slice pairs (2p, 2p+1).  For each pair the
-    // workgroup stages in LDS the two slices, the +x/+y neighbour voxels next
-    // to them and the three shadow-map rows the pair's records sample; scans
-    // the counts of the pair's level-0 rows and of the level-1 rows of L1
-    // slice p (whose colours come from these two slices); expands their set
-    // bits into a list of cells; and writes one record per thread, in order,
-    // with no global access but the stores.  The next pair's loads are in
-    // flight meanwhile.  Levels 2..4 follow level by level, reading colours
-    // and shadows from global memory (a few percent of the records).
-    uint8_t *scr = reinterpret_cast<uint8_t *>(lds + L::SCR);
-    const uint32_t *soff = reinterpret_cast<const uint32_t *>(scr + L::O_SOFF);
-    uint8_t *stg = scr + L::O_STG;
-    uint16_t *rowp = reinterpret_cast<uint16_t *>(scr + L::O_ROWP);
-    uint16_t *list = reinterpret_cast<uint16_t *>(scr + L::O_LIST);
-    uint64_t *out = a.splats + base;
-    constexpr int NP = R / 2 + 1, n1 = G::n(1);
-    constexpr int SEGW = R < 16 ? R : 16;             // bits per expansion unit
-    constexpr int NSEG0 = R / SEGW + 1;              // level-0 row: segments + the x = R cell
-    constexpr int NSEG1 = (n1 + 15) / 16;
-    PairRegs<B, BS> pr;
-    pair_load<B, BS>(a, c, 0, pr);
-    for (int p = 0; p < NP; p++) {
-        const int z0 = 2 * p;
-        const int nrow0 = (z0 + 1 <= R ? 2 : 1) * N0;
-        const int nrows = nrow0 + (a.lod ? n1 : 0);
-        __syncthreads();                      // the previous pair is done with the scratch
-        pair_store<B, BS>(pr, stg);
-        auto row_bits = [&](int i) -> uint64_t {
-            return i < nrow0 ? lds[L::V0 + z0 * N0 + i] : lds[L::P1 + p * n1 + (i - nrow0)];
-        };
-        auto row_count = [&](int i) -> uint32_t {
-            if (i < nrow0) {
-                const int z = z0 + i / N0, y = i % N0;
-                uint32_t cv = __popcll(lds[L::V0 + z * N0 + y]);
-                if (z < R && y < R) cv += (uint32_t)(lds[L::XB + z] >> y) & 1u;
-                return cv;
-            }
-            return (uint32_t)__popcll(lds[L::P1 + p * n1 + (i - nrow0)]);
-        };
-        const uint32_t npair = scan_rows<BS>(nrows, row_count, rowp, wsum);
-        if (p + 1 < NP) pair_load<B, BS>(a, c, z0 + 2, pr);
-        const uint32_t nl0 = nrow0 < nrows ? (uint32_t)rowp[nrow0] : npair;
-        // expansion: one unit per (row, 16-bit segment)
-        const int nunits = nrow0 * NSEG0 + (nrows - nrow0) * NSEG1;
-        for (int u = threadIdx.x; u < nunits; u += BS) {
-            if (u < nrow0 * NSEG0) {
-                const int i = u / NSEG0, sg = u % NSEG0;
-                const int sidx = i / N0, y = i % N0, z = z0 + sidx;
-                const uint64_t rb = row_bits(i);
-                const uint32_t code0 = ((uint32_t)sidx << 14) | ((uint32_t)y << 7);
-                if (sg == NSEG0 - 1) {             // the cell x = R of this row
-                    if (z < R && y < R && ((lds[L::XB + z] >> y) & 1ull))
-                        list[rowp[i] + __popcll(rb)] = (uint16_t)(code0 | (uint32_t)R);
-                } else {
-                    uint32_t pos = rowp[i] + (sg ? (uint32_t)__popcll(rb & ((1ull << (sg * SEGW)) - 1ull)) : 0u);
-                    uint32_t seg = (uint32_t)(rb >> (sg * SEGW)) & ((SEGW == 32 ? 0u : (1u << SEGW)) - 1u);
-                    while (seg) {
-                        const uint32_t x = (uint32_t)(sg * SEGW) + (uint32_t)__builtin_ctz(seg);
-                        seg &= seg - 1u;
-                        list[pos++] = (uint16_t)(code0 | x);
-                    }
-                }
-            } else {
-                const int v = u - nrow0 * NSEG0, i = nrow0 + v / NSEG1, sg = v % NSEG1;
-                const int y = i - nrow0;
-                const uint64_t rb = row_bits(i);
-                uint32_t pos = rowp[i] + (sg ? (uint32_t)__popcll(rb & ((1ull << (sg * 16)) - 1ull)) : 0u);
-                uint32_t seg = (uint32_t)(rb >> (sg * 16)) & 0xFFFFu;
-                while (seg) {
-                    const uint32_t x = (uint32_t)(sg * 16) + (uint32_t)__builtin_ctz(seg);
-                    seg &= seg - 1u;
-                    list[pos++] = (uint16_t)(0x8000u | ((uint32_t)y << 7) | x);
-                }
-            }
-        }
-        // Wait for the next pair's prefetch here (vmcnt 0; the encoding leaves
-        // the other counters free), before this pair's stores are issued:
-        // vmcnt retires loads and stores in issue order, so a wait for the
-        // prefetch after the stores would also wait for the stores.
-        __builtin_amdgcn_s_waitcnt(0x0F70);
-        __syncthreads();
-        const uint32_t o0 = soff[L::SLB0 + z0], o1 = soff[L::SLB1 + p];
-        for (uint32_t j = threadIdx.x; j < npair; j += BS) {
-            const uint32_t e = list[j];
-            const int x = (int)(e & 0x7Fu), y = (int)((e >> 7) & 0x7Fu);
-            if (j < nl0) {
-                const int z = z0 + (int)((e >> 14) & 1u);
-                const uint32_t col = staged_colour<B>(stg, x, y, z, z0);
-                const uint32_t wy = c.org[1] + y;
-                const uint32_t sh = staged_shadow<B>(stg, z - z0, x + y, wy);
-                uint32_t fl = 0;
-                if constexpr (V2) fl = face_flags<B, V2>(lds, x, y, z);
-                out[o0 + j] = pack_record<V2>(c.org[0] + x, wy, c.org[2] + z, col, sh, fl);
-            } else {
-                const int yy = (int)((e >> 7) & 0x3Fu);
-                int qx = x, qy = yy, qz = p;
-                descend<B, V2>(lds, 1, qx, qy, qz);
-                const uint32_t col = staged_colour<B>(stg, qx, qy, qz, z0);
-                const uint32_t wy = c.org[1] + 2u * yy;
-                const uint32_t sh = staged_shadow<B>(stg, 2, 2 * (x + yy) + 4, wy + 2u);
-                uint32_t fl = 0;
-                if constexpr (V2) fl = face_flags<B, V2>(lds, qx, qy, qz);
-                out[o1 + (j - nl0)] = pack_record<V2>(c.org[0] + 2u * x, wy, c.org[2] + 2u * p, col,
-                                                      sh, fl);
-            }
-        }
-    }
-    if (!a.lod) return;
-    // levels 2..4: one level at a time, (z, y, x) of each cell in the list
-    for (int l = 2; l <= 4; l++) {
-        const int nl = G::n(l), nrows = nl * nl;
-        const uint64_t *prow = lds + L::P1 + (G::row_base(l) - G::RB1);
-        __syncthreads();
-        auto row_count = [&](int i) -> uint32_t { return (uint32_t)__popcll(prow[i]); };
-        const uint32_t nlev = scan_rows<BS>(nrows, row_count, rowp, wsum);
-        for (int i = threadIdx.x; i < nrows; i += BS) {
-            uint64_t rb = prow[i];
-            uint32_t pos = rowp[i];
-            const uint32_t zy = ((uint32_t)(i / nl) << 10) | ((uint32_t)(i % nl) << 5);
-            while (rb) {
-                const uint32_t x = (uint32_t)__builtin_ctzll(rb);
-                rb &= rb - 1ull;
-                list[pos++] = (uint16_t)(zy | x);
-            }
-        }
-        __syncthreads();
-        const uint32_t ol = soff[l == 2 ? L::SLB2 : l == 3 ? L::SLB3 : L::SLB4];
-        const uint32_t d = 1u << l;
-        for (uint32_t j = threadIdx.x; j < nlev; j += BS) {
-            const uint32_t e = list[j];
-            const int x = (int)(e & 31u), y = (int)((e >> 5) & 31u), z = (int)(e >> 10);
-            int qx = x, qy = y, qz = z;
-            descend<B, V2>(lds, l, qx, qy, qz);
-            const uint32_t col = l0_colour<B>(c, qx, qy, qz);
-            const uint32_t wx = c.org[0] + ((uint32_t)x << l);
-            const uint32_t wy = c.org[1] + ((uint32_t)y << l);
-            const uint32_t wz = c.org[2] + ((uint32_t)z << l);
-            const uint32_t sh = shadow_bit(a.map, a.sx, wx + d, wy + d, wz + d);
-            uint32_t fl = 0;
-            if constexpr (V2) fl = face_flags<B, V2>(lds, qx, qy, qz);
-            out[ol + j] = pack_record<V2>(wx, wy, wz, col, sh, fl);
-        }
-    }
+    // ---- 5. emission: waves take z-slices from the work counter ----
+    emit_slices<B, BS, V2, U>(a, c, lds, a.splats + base);
 }
 
-// LDS (~79 KiB) allows two workgroups per CU; ask the register allocator for
-// the matching occupancy (512 threads: 4 waves per SIMD, at most 128 VGPRs;
-// 1024 threads: one workgroup of 4 waves per SIMD).
-template <int B, int BS, bool V2, bool MASK_OUT>
+// LDS (about 67 KiB at 512 threads for R = 64) allows two workgroups per CU;
+// ask the register allocator for the matching occupancy (512 threads: 4 waves
+// per SIMD, at most 128 VGPRs; 1024 threads: one workgroup of 4 waves per SIMD).
+// U: records each lane prepares before its stores in the emission.
+template <int B, int BS, bool V2, bool MASK_OUT, int U>
 __global__ void __launch_bounds__(BS)
 __attribute__((amdgpu_waves_per_eu(BS == 256 ? 2 : 4))) k_cull(CullArgs a)
 {
-    __shared__ uint64_t lds[Lay<B, V2>::TOTAL];
+    __shared__ uint64_t lds[Lay<B, BS, V2>::TOTAL];
     __shared__ uint32_t wsum[BS / 64 + 1];
     __shared__ uint32_t s_ticket;
     __shared__ uint64_t s_base;
     if (a.single) {
-        cull_chunk<B, BS, V2, MASK_OUT>(a, 0u, 1u, lds, wsum, &s_base);
+        cull_chunk<B, BS, V2, MASK_OUT, U>(a, 0u, 1u, lds, wsum, &s_base);
         return;
     }
     if (MASK_OUT) {   // make_mask: one workgroup per listed chunk, no offsets
         for (uint32_t k = blockIdx.x; k < a.n; k += gridDim.x) {
-            cull_chunk<B, BS, V2, MASK_OUT>(a, k, a.n, lds, wsum, &s_base);
+            cull_chunk<B, BS, V2, MASK_OUT, U>(a, k, a.n, lds, wsum, &s_base);
             __syncthreads();
         }
         return;
@@ -987,7 +1042,7 @@ __attribute__((amdgpu_waves_per_eu(BS == 256 ? 2 : 4))) k_cull(CullArgs a)
         __syncthreads();
         const uint32_t k = uni(s_ticket);
         if (k >= n) break;
-        cull_chunk<B, BS, V2, MASK_OUT>(a, k, n, lds, wsum, &s_base);
+        cull_chunk<B, BS, V2, MASK_OUT, U>(a, k, n, lds, wsum, &s_base);
         __syncthreads();
     }
 }
@@ -1002,39 +1057,61 @@ __global__ void k_cull_reset(uint32_t *ticket, uint64_t *state, uint32_t n, uint
     for (uint32_t j = i; j < n; j += gridDim.x * blockDim.x) state[j] = 0ull;
 }
 
-template <int B, int BS, bool V2, bool MASK_OUT>
-static hipError_t launch_cull(const CullArgs &a, int grid, hipStream_t st)
+// A kernel instantiation: launch it, or ask how many of its workgroups fit
+// on one CU (registers and LDS).
+struct CullLaunch {
+    const CullArgs *a;      // launch when set, else query residency
+    int grid;
+    hipStream_t st;
+    int *per_cu;
+};
+
+template <int B, int BS, bool V2, bool MASK_OUT, int U>
+static hipError_t launch_cull(const CullLaunch &L)
 {
-    hipLaunchKernelGGL((k_cull<B, BS, V2, MASK_OUT>), dim3(grid), dim3(BS), 0, st, a);
+    if (!L.a) {
+        return hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            L.per_cu, reinterpret_cast<const void *>(&k_cull<B, BS, V2, MASK_OUT, U>), BS, 0);
+    }
+    hipLaunchKernelGGL((k_cull<B, BS, V2, MASK_OUT, U>), dim3(L.grid), dim3(BS), 0, L.st, *L.a);
     return hipGetLastError();
 }
 
+// Records in flight per lane: 2 or 4 for R = 64 (a tuning choice), 4 below.
+template <int B, int BS, bool V2, bool MASK_OUT>
+static hipError_t dispatch_u(const CullLaunch &L, int u)
+{
+    if constexpr (B == 6 && !MASK_OUT)
+        if (u == 2) return launch_cull<B, BS, V2, MASK_OUT, 2>(L);
+    return launch_cull<B, BS, V2, MASK_OUT, 4>(L);
+}
+
 template <int B, bool MASK_OUT>
-static hipError_t dispatch_bs(const CullArgs &a, bool v2, int bs, int grid, hipStream_t st)
+static hipError_t dispatch_bs(const CullLaunch &L, bool v2, int bs, int u)
 {
     if (v2) {
         switch (bs) {
-        case 256: return launch_cull<B, 256, true, MASK_OUT>(a, grid, st);
-        case 1024: return launch_cull<B, 1024, true, MASK_OUT>(a, grid, st);
-        default: return launch_cull<B, 512, true, MASK_OUT>(a, grid, st);
+        case 256: return dispatch_u<B, 256, true, MASK_OUT>(L, u);
+        case 1024: return dispatch_u<B, 1024, true, MASK_OUT>(L, u);
+        default: return dispatch_u<B, 512, true, MASK_OUT>(L, u);
         }
     }
     switch (bs) {
-    case 256: return launch_cull<B, 256, false, MASK_OUT>(a, grid, st);
-    case 1024: return launch_cull<B, 1024, false, MASK_OUT>(a, grid, st);
-    default: return launch_cull<B, 512, false, MASK_OUT>(a, grid, st);
+    case 256: return dispatch_u<B, 256, false, MASK_OUT>(L, u);
+    case 1024: return dispatch_u<B, 1024, false, MASK_OUT>(L, u);
+    default: return dispatch_u<B, 512, false, MASK_OUT>(L, u);
     }
 }
 
 template <bool MASK_OUT>
-static hipError_t dispatch(int b, const CullArgs &a, bool v2, int bs, int grid, hipStream_t st)
+static hipError_t dispatch(int b, const CullLaunch &L, bool v2, int bs, int u)
 {
     switch (b) {
-    case 2: return dispatch_bs<2, MASK_OUT>(a, v2, bs, grid, st);
-    case 3: return dispatch_bs<3, MASK_OUT>(a, v2, bs, grid, st);
-    case 4: return dispatch_bs<4, MASK_OUT>(a, v2, bs, grid, st);
-    case 5: return dispatch_bs<5, MASK_OUT>(a, v2, bs, grid, st);
-    default: return dispatch_bs<6, MASK_OUT>(a, v2, bs, grid, st);
+    case 2: return dispatch_bs<2, MASK_OUT>(L, v2, bs, u);
+    case 3: return dispatch_bs<3, MASK_OUT>(L, v2, bs, u);
+    case 4: return dispatch_bs<4, MASK_OUT>(L, v2, bs, u);
+    case 5: return dispatch_bs<5, MASK_OUT>(L, v2, bs, u);
+    default: return dispatch_bs<6, MASK_OUT>(L, v2, bs, u);
     }
 }
 
@@ -1091,7 +1168,8 @@ int cull_mesh_launch(vp_ctx *ctx, const vp_world_desc *world, const uint8_t *d_v
     a.level_counts = out->d_level_counts;
     a.status = ctx->d_status;
     a.nchunks = (uint32_t)(1ull << (world->max_bitw[0] + world->max_bitw[1] + world->max_bitw[2]));
-    int grid = 1, bs = ctx->cull_block;
+    const vp_remesh_shape_i sh = remesh_shape(ctx, world);
+    int grid = 1, bs = sh.block;
     // one 64^3 chunk alone: a 1024-thread workgroup (16 waves on one CU)
     // finishes it in 0.70x the time of the batched path's 512-thread shape
     if (a.single && world->root_bitw == 6) bs = 1024;
@@ -1099,14 +1177,15 @@ int cull_mesh_launch(vp_ctx *ctx, const vp_world_desc *world, const uint8_t *d_v
         if (!ticket || !state) return VP_ERR_ARG;
         a.ticket = ticket;
         a.state = state;
-        const int resident = ctx->num_cus * ctx->cull_wg_per_cu;
+        const int resident = ctx->num_cus * sh.wg_per_cu;
         grid = (int)((uint64_t)n < (uint64_t)resident ? n : (uint32_t)resident);
         const int rgrid = (int)((n + 255u) / 256u);
         hipLaunchKernelGGL(k_cull_reset, dim3(rgrid < 1024 ? rgrid : 1024), dim3(256), 0, st,
                            a.ticket, a.state, n, a.offsets);
         VP_HIP_TRY(hipGetLastError());
     }
-    VP_HIP_TRY(dispatch<false>(world->root_bitw, a, (flags & VP_CM_RECORD_V2) != 0, bs, grid, st));
+    const CullLaunch L = {&a, grid, st, nullptr};
+    VP_HIP_TRY(dispatch<false>(world->root_bitw, L, (flags & VP_CM_RECORD_V2) != 0, bs, sh.u));
     return VP_OK;
 }
 
@@ -1156,7 +1235,22 @@ extern "C" int vp_gpu_make_mask(vp_ctx *ctx, const vp_world_desc *world,
     a.nchunks = (uint32_t)(1ull << (world->max_bitw[0] + world->max_bitw[1] + world->max_bitw[2]));
     const int resident = ctx->num_cus * 2;
     const int grid = (int)(n < (uint32_t)resident ? n : (uint32_t)resident);
-    VP_HIP_TRY(dispatch<true>(world->root_bitw, a, false, 512, grid, as_stream(stream)));
+    const CullLaunch L = {&a, grid, as_stream(stream), nullptr};
+    VP_HIP_TRY(dispatch<true>(world->root_bitw, L, false, 512, 4));
+    return VP_OK;
+}
+
+extern "C" int vp_ctx_remesh_resident(vp_ctx *ctx, const vp_world_desc *world, uint32_t flags,
+                                      const vp_remesh_shape *shape, int *per_cu)
+{
+    if (!ctx || !shape || !per_cu) return VP_ERR_ARG;
+    int rc = check_world(world);
+    if (rc) return rc;
+    const int b = shape->block, u = shape->records_in_flight;
+    if ((b != 256 && b != 512 && b != 1024) || (u != 2 && u != 4)) return VP_ERR_ARG;
+    *per_cu = 0;
+    const CullLaunch L = {nullptr, 0, nullptr, per_cu};
+    VP_HIP_TRY(dispatch<false>(world->root_bitw, L, (flags & VP_CM_RECORD_V2) != 0, b, u));
     return VP_OK;
 }
 

@@ -534,13 +534,13 @@ __global__ void __launch_bounds__(kMeshBS, 4) k_mesh(MeshArgs a)
                 const uint32_t fl = lo + j;                     // chunk-local face index
                 const uint64_t gf = base + fl;
                 uint4 *gp = reinterpret_cast<uint4 *>(a.geom + 16 * gf);
-                gp[0] = make_uint4(wd[0], wd[1], wd[2], wd[3]);
-                gp[1] = make_uint4(wd[4], wd[5], wd[6], wd[7]);
+                nt_store16(gp, make_uint4(wd[0], wd[1], wd[2], wd[3]));
+                nt_store16(gp + 1, make_uint4(wd[4], wd[5], wd[6], wd[7]));
                 const uint32_t vb = 4u * fl;
                 uint2 *ip = reinterpret_cast<uint2 *>(a.index + 6 * gf);
-                ip[0] = make_uint2(vb + m_tri(normal, rot, 0), vb + m_tri(normal, rot, 1));
-                ip[1] = make_uint2(vb + m_tri(normal, rot, 2), vb + m_tri(normal, rot, 3));
-                ip[2] = make_uint2(vb + m_tri(normal, rot, 4), vb + m_tri(normal, rot, 5));
+                nt_store8(ip, make_uint2(vb + m_tri(normal, rot, 0), vb + m_tri(normal, rot, 1)));
+                nt_store8(ip + 1, make_uint2(vb + m_tri(normal, rot, 2), vb + m_tri(normal, rot, 3)));
+                nt_store8(ip + 2, make_uint2(vb + m_tri(normal, rot, 4), vb + m_tri(normal, rot, 5)));
                 __builtin_amdgcn_sched_barrier(0);   // one face at a time: bounded live ranges
             }
             __syncthreads();

@@ -411,12 +411,391 @@ __global__ void __launch_bounds__(kBlock) k_raster_blocks(const int16_t *splats,
                                                            const float4 *rays, uint64_t *frame,
                                                            const uint32_t *order,
                                                            const uint32_t *cnt,
-                                                           const uint64_t *d_n)
+                                                           const uint64_t *d_n,
+                                                           const uint32_t *tiles_ok)
 {
+    if (tiles_ok && *tiles_ok) return;    // the screen-tile path resolved this frame
     const uint32_t total = cnt[kTotal];
     if (d_n) n = min(*d_n, n);
     for (uint32_t t = blockIdx.x; t < total; t += gridDim.x) {
         const uint64_t i = (uint64_t)order[t] * kBlock + threadIdx.x;
+        raster_splat(splats, lod, lod_log2, n, i, c, rays, frame);
+    }
+}
+
+// ---- screen-tile resolve ------------------------------------------------------
+// The visible blocks' splats are binned into kTile x kTile pixel tiles; each
+// tile is then resolved by one workgroup with its rays and frame keys in LDS,
+// so a fragment costs LDS reads and one LDS atomicMin instead of global
+// gathers and an L2 atomic.  A splat whose pixel box is at most kTileSide
+// pixels on a side goes into each tile it touches (at most four entries); a
+// larger one -- a near splat, few of them, many pixels each -- goes to the big
+// list, which is drawn first against the frame in device memory, spread over
+// the whole grid.  The tiles then start from those keys, so the near
+// occluders reject the small splats behind them from the first entry on.
+// The keys are the same (depth bits << 32 | id) and min is order-free, so the
+// frame is identical to the global path's.  If a frame needs more entries
+// than the buffers hold, the scan leaves the flag clear and the global path
+// (k_raster_blocks) draws the whole frame instead.
+constexpr int kTileLog = 5, kTile = 1 << kTileLog, kTilePx = kTile * kTile;
+constexpr int kTileSide = 8;                  // larger pixel boxes go to the big list
+static_assert(kTilePx % kBlock == 0, "k_tile_raster: whole pixels per thread");
+constexpr uint32_t kMaxTiles = 8192;          // LDS bins of the binning kernels: 32 KiB
+constexpr uint32_t kItem = 8192;              // tile entries per raster work item
+constexpr uint32_t kSingle = 1u << 31;        // item flag: the tile's only item
+// tile counters (u32): [0] raster items, [1] entries, [2] big splats counted,
+// [3] flag: the tile path draws this frame, [4] big-list cursor
+constexpr int kTcItems = 0, kTcPairs = 1, kTcBig = 2, kTcOk = 3, kTcBigCur = 4, kTcWords = 8;
+
+struct TileBufs {
+    uint32_t tx, ty, nt;      // tiles across, down, total
+    uint32_t *hist;           // [G][nt] per-workgroup entry counts
+    uint32_t *cnt;            // [nt] entries per tile
+    uint32_t *cur;            // [nt] placement cursors
+    uint32_t *tc;             // kTcWords counters
+    uint4    *items;          // (tile | kSingle, begin, end, 0)
+    uint32_t  item_cap;
+    uint32_t *pairs;          // splat index per entry, tile-major
+    uint32_t  pair_cap;
+    uint32_t *big;            // splat indices of the big list
+    uint32_t  big_cap;
+};
+
+// Vertex stage of splat i (i < n), and the tiles its pixel box touches:
+// 0 none, 1 tile entries (t0 = first tile, span bits: +1 tile in x, +1 in y),
+// 2 big.
+__device__ __forceinline__ int splat_tiles(const int16_t *splats, const uint8_t *lod,
+                                           uint32_t lod_log2, uint64_t i, const SplatCam &c,
+                                           uint32_t tx, VS &vs, uint32_t &t0, uint32_t &span)
+{
+    const float lodf = (float)(1u << (lod ? lod[i] : lod_log2));
+    if (!vertex_stage(splats + 4 * i, lodf, c, vs)) return 0;
+    if (vs.x0 > vs.x1 || vs.y0 > vs.y1) return 0;
+    if (vs.x1 - vs.x0 >= kTileSide || vs.y1 - vs.y0 >= kTileSide) return 2;
+    // a box of at most kTileSide <= kTile pixels touches at most 2 x 2 tiles
+    const uint32_t a = (uint32_t)vs.x0 >> kTileLog, b = (uint32_t)vs.x1 >> kTileLog;
+    const uint32_t d = (uint32_t)vs.y0 >> kTileLog, e = (uint32_t)vs.y1 >> kTileLog;
+    t0 = d * tx + a;
+    span = (b - a) | ((e - d) << 1);
+    return 1;
+}
+
+__global__ void k_tile_reset(uint32_t *cnt, uint32_t nt, uint32_t *tc)
+{
+    for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < nt; t += gridDim.x * blockDim.x)
+        cnt[t] = 0u;
+    if (blockIdx.x == 0 && threadIdx.x < kTcWords) tc[threadIdx.x] = 0u;
+}
+
+// Workgroup g takes the ordered visible blocks g, g + G, ... (the same
+// assignment as k_tile_scatter) and counts its entries per tile in LDS; the
+// counts go to its row of the histogram and are added to the tile totals.
+__global__ void __launch_bounds__(kBlock) k_tile_count(const int16_t *splats, const uint8_t *lod,
+                                                       uint32_t lod_log2, uint64_t n, SplatCam c,
+                                                       const uint32_t *order, const uint32_t *cnt,
+                                                       const uint64_t *d_n, TileBufs tb)
+{
+    extern __shared__ uint32_t hist[];
+    for (uint32_t t = threadIdx.x; t < tb.nt; t += blockDim.x) hist[t] = 0u;
+    __syncthreads();
+    const uint32_t total = cnt[kTotal];
+    if (d_n) n = min(*d_n, n);
+    uint32_t nbig = 0;
+    for (uint32_t b = blockIdx.x; b < total; b += gridDim.x) {
+        const uint64_t i = (uint64_t)order[b] * kBlock + threadIdx.x;
+        if (i >= n) continue;
+        VS vs;
+        uint32_t t0 = 0, span = 0;
+        const int k = splat_tiles(splats, lod, lod_log2, i, c, tb.tx, vs, t0, span);
+        if (k == 2) nbig++;
+        if (k != 1) continue;
+        atomicAdd(hist + t0, 1u);
+        if (span & 1u) atomicAdd(hist + t0 + 1, 1u);
+        if (span & 2u) atomicAdd(hist + t0 + tb.tx, 1u);
+        if (span == 3u) atomicAdd(hist + t0 + tb.tx + 1, 1u);
+    }
+    __syncthreads();
+    uint32_t *row = tb.hist + (size_t)blockIdx.x * tb.nt;
+    for (uint32_t t = threadIdx.x; t < tb.nt; t += blockDim.x) {
+        const uint32_t v = hist[t];
+        row[t] = v;
+        if (v) atomicAdd(tb.cnt + t, v);
+    }
+    for (int d = 32; d >= 1; d >>= 1) nbig += (uint32_t)__shfl_xor((int)nbig, d, 64);
+    if ((threadIdx.x & 63u) == 0 && nbig) atomicAdd(tb.tc + kTcBig, nbig);
+}
+
+// One workgroup: tile offsets (exclusive scan of the counts), placement
+// cursors, the raster items (runs of at most kItem entries of one tile) and
+// the flag that hands the frame to the tile path when everything fits.
+constexpr int kScanThreads = 1024;
+__device__ __forceinline__ uint64_t shfl_up64(uint64_t v, int d)
+{
+    const uint32_t lo = (uint32_t)__shfl_up((int)(uint32_t)v, d, 64);
+    const uint32_t hi = (uint32_t)__shfl_up((int)(uint32_t)(v >> 32), d, 64);
+    return ((uint64_t)hi << 32) | lo;
+}
+__global__ void __launch_bounds__(kScanThreads) k_tile_scan(TileBufs tb)
+{
+    __shared__ uint64_t wsum[2][kScanThreads / 64];
+    __shared__ uint32_t s_ok;
+    const uint32_t per = (tb.nt + kScanThreads - 1) / kScanThreads;
+    const uint32_t t0 = min(threadIdx.x * per, tb.nt), t1 = min(t0 + per, tb.nt);
+    // entry totals in 64 bits: four entries per splat can pass 2^32
+    uint64_t sc = 0, si = 0;
+    for (uint32_t t = t0; t < t1; t++) {
+        const uint32_t v = tb.cnt[t];
+        sc += v;
+        si += (v + kItem - 1) / kItem;
+    }
+    // block-wide exclusive scans of (entries, items)
+    const uint32_t lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
+    uint64_t ic = sc, ii = si;
+    for (int d = 1; d < 64; d <<= 1) {
+        const uint64_t a = shfl_up64(ic, d), b = shfl_up64(ii, d);
+        if ((int)lane >= d) { ic += a; ii += b; }
+    }
+    if (lane == 63) { wsum[0][w] = ic; wsum[1][w] = ii; }
+    __syncthreads();
+    uint64_t bc = 0, bi = 0, tc = 0, ti = 0;
+    for (uint32_t k = 0; k < kScanThreads / 64; k++) {
+        if (k < w) { bc += wsum[0][k]; bi += wsum[1][k]; }
+        tc += wsum[0][k];
+        ti += wsum[1][k];
+    }
+    if (threadIdx.x == 0) {
+        const uint32_t big = tb.tc[kTcBig];
+        const bool ok = tc <= tb.pair_cap && ti <= tb.item_cap && big <= tb.big_cap;
+        s_ok = ok ? 1u : 0u;
+        tb.tc[kTcPairs] = (uint32_t)min(tc, (uint64_t)0xFFFFFFFFu);
+        tb.tc[kTcItems] = ok ? (uint32_t)ti : 0u;
+        tb.tc[kTcOk] = s_ok;
+    }
+    __syncthreads();
+    if (!s_ok) return;
+    // every total now fits in 32 bits
+    uint32_t off = (uint32_t)(bc + ic - sc), io = (uint32_t)(bi + ii - si);
+    for (uint32_t t = t0; t < t1; t++) {
+        const uint32_t v = tb.cnt[t];
+        tb.cur[t] = off;
+        const uint32_t k = (v + kItem - 1) / kItem;
+        for (uint32_t q = 0; q < k; q++) {
+            const uint32_t b = off + q * kItem, e = min(b + kItem, off + v);
+            tb.items[io + q] = make_uint4(t | (k == 1 ? kSingle : 0u), b, e, 0u);
+        }
+        off += v;
+        io += k;
+    }
+}
+
+// The same blocks as k_tile_count: reserve the workgroup's range of every
+// tile it touches, then place each entry (LDS cursors); big splats are
+// appended to the big list.
+__global__ void __launch_bounds__(kBlock) k_tile_scatter(const int16_t *splats, const uint8_t *lod,
+                                                         uint32_t lod_log2, uint64_t n, SplatCam c,
+                                                         const uint32_t *order, const uint32_t *cnt,
+                                                         const uint64_t *d_n, TileBufs tb)
+{
+    extern __shared__ uint32_t lcur[];
+    if (!tb.tc[kTcOk]) return;
+    const uint32_t *row = tb.hist + (size_t)blockIdx.x * tb.nt;
+    for (uint32_t t = threadIdx.x; t < tb.nt; t += blockDim.x) {
+        const uint32_t v = row[t];
+        lcur[t] = v ? atomicAdd(tb.cur + t, v) : 0u;
+    }
+    __syncthreads();
+    const uint32_t total = cnt[kTotal];
+    if (d_n) n = min(*d_n, n);
+    const uint32_t lane = threadIdx.x & 63u;
+    for (uint32_t b = blockIdx.x; b < total; b += gridDim.x) {
+        const uint64_t i = (uint64_t)order[b] * kBlock + threadIdx.x;
+        VS vs;
+        uint32_t t0 = 0, span = 0;
+        const int k = i < n ? splat_tiles(splats, lod, lod_log2, i, c, tb.tx, vs, t0, span) : 0;
+        if (k == 1) {
+            uint32_t q = atomicAdd(lcur + t0, 1u);
+            if (q < tb.pair_cap) tb.pairs[q] = (uint32_t)i;
+            if (span & 1u) {
+                q = atomicAdd(lcur + t0 + 1, 1u);
+                if (q < tb.pair_cap) tb.pairs[q] = (uint32_t)i;
+            }
+            if (span & 2u) {
+                q = atomicAdd(lcur + t0 + tb.tx, 1u);
+                if (q < tb.pair_cap) tb.pairs[q] = (uint32_t)i;
+            }
+            if (span == 3u) {
+                q = atomicAdd(lcur + t0 + tb.tx + 1, 1u);
+                if (q < tb.pair_cap) tb.pairs[q] = (uint32_t)i;
+            }
+        }
+        const uint64_t m = __ballot(k == 2);
+        if (m) {
+            uint32_t base = 0;
+            if (lane == (uint32_t)__builtin_ctzll(m))
+                base = atomicAdd(tb.tc + kTcBigCur, (uint32_t)__popcll(m));
+            base = (uint32_t)__shfl((int)base, __builtin_ctzll(m), 64);
+            if (k == 2) {
+                const uint32_t q = base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+                if (q < tb.big_cap) tb.big[q] = (uint32_t)i;
+            }
+        }
+    }
+}
+
+// One tile's fragment in LDS: lp indexes the tile's pixel.
+__device__ __forceinline__ void fragment_lds(const float4 *lray, uint64_t *lkey, float far_plane,
+                                             float vx, float vy, float vz, float half, uint32_t id,
+                                             uint64_t lb, int lp)
+{
+    const uint64_t cur = lkey[lp];
+    if (lb > cur) return;
+    const float4 inv = lray[lp];
+    float t0, t1;
+    aabb(vx, vy, vz, half, inv.x, inv.y, inv.z, t0, t1);
+    if (!(t0 <= t1)) return;
+    float d = t0 / far_plane;
+    if (!(d < 1.0f)) return;
+    if (d <= 0.0f) d = 0.0f;
+    const uint64_t key = ((uint64_t)__float_as_uint(d) << 32) | id;
+    if (key < cur) atomicMin(reinterpret_cast<unsigned long long *>(lkey + lp), key);
+}
+
+// Entry i (i >= n: none) of one lane, clipped to the tile at (px0, py0);
+// every lane of the wave calls it.  As raster_splat: boxes of up to
+// kSmallArea pixels by their own lane, larger ones by the whole wave.
+__device__ __forceinline__ void tile_splat(const int16_t *splats, const uint8_t *lod,
+                                           uint32_t lod_log2, uint64_t n, uint64_t i,
+                                           const SplatCam &c, int px0, int py0,
+                                           const float4 *lray, uint64_t *lkey)
+{
+    VS vs;
+    bool ok = false;
+    if (i < n) {
+        const float lodf = (float)(1u << (lod ? lod[i] : lod_log2));
+        ok = vertex_stage(splats + 4 * i, lodf, c, vs);
+    }
+    int x0 = 0, x1 = -1, y0 = 0, y1 = -1;
+    if (ok) {
+        x0 = vs.x0 > px0 ? vs.x0 : px0;
+        x1 = vs.x1 < px0 + kTile - 1 ? vs.x1 : px0 + kTile - 1;
+        y0 = vs.y0 > py0 ? vs.y0 : py0;
+        y1 = vs.y1 < py0 + kTile - 1 ? vs.y1 : py0 + kTile - 1;
+    }
+    int area = 0;
+    if (x0 <= x1 && y0 <= y1) area = (x1 - x0 + 1) * (y1 - y0 + 1);
+    const uint32_t id = (uint32_t)i;
+    if (area > 0 && area <= kSmallArea) {
+        for (int py = y0; py <= y1; py++) {
+            const float cy = (float)py + 0.5f;
+            if (!(vs.loy <= cy && cy < vs.hiy)) continue;
+            for (int px = x0; px <= x1; px++) {
+                const float cx = (float)px + 0.5f;
+                if (!(vs.lox <= cx && cx < vs.hix)) continue;
+                fragment_lds(lray, lkey, c.far_plane, vs.vx, vs.vy, vs.vz, vs.half, id, vs.lb,
+                             ((py - py0) << kTileLog) + (px - px0));
+            }
+        }
+    }
+    uint64_t big = __ballot(area > kSmallArea);
+    const int lane = (int)(threadIdx.x & 63u);
+    while (big) {
+        const int src = __builtin_ctzll(big);
+        big &= big - 1ull;
+        const float vx = __shfl(vs.vx, src, 64), vy = __shfl(vs.vy, src, 64);
+        const float vz = __shfl(vs.vz, src, 64), half = __shfl(vs.half, src, 64);
+        const float lox = __shfl(vs.lox, src, 64), hix = __shfl(vs.hix, src, 64);
+        const float loy = __shfl(vs.loy, src, 64), hiy = __shfl(vs.hiy, src, 64);
+        const int bx0 = __shfl(x0, src, 64), bx1 = __shfl(x1, src, 64);
+        const int by0 = __shfl(y0, src, 64);
+        const int a = __shfl(area, src, 64);
+        const uint32_t sid = __shfl(id, src, 64);
+        const uint64_t lb = ((uint64_t)(uint32_t)__shfl((int)(uint32_t)(vs.lb >> 32), src, 64)) << 32;
+        // lane k takes pixels k, k + 64, ... of the clipped box, row by row
+        // (w <= kTile, so 64 / w and the row of k come exactly from the
+        // approximate reciprocal, as in raster_splat)
+        const int w = bx1 - bx0 + 1;
+        const float rw = __builtin_amdgcn_rcpf((float)w);
+        const int dq = (int)(64.5f * rw), dr = 64 - dq * w;
+        int row = (int)(((float)lane + 0.5f) * rw), col = lane - row * w;
+        for (int k = lane; k < a; k += 64) {
+            const int px = bx0 + col, py = by0 + row;
+            col += dr;
+            row += dq;
+            if (col >= w) { col -= w; row += 1; }
+            const float cx = (float)px + 0.5f, cy = (float)py + 0.5f;
+            if (!(lox <= cx && cx < hix && loy <= cy && cy < hiy)) continue;
+            fragment_lds(lray, lkey, c.far_plane, vx, vy, vz, half, sid, lb,
+                         ((py - py0) << kTileLog) + (px - px0));
+        }
+    }
+}
+
+// Raster items, strided over the grid: a tile's rays in LDS and its keys as
+// the big list left them, its entries in kBlock-wide steps, then its keys out
+// -- stored when the item is the tile's only one, else merged with a global
+// atomicMin where the item lowered a key.
+__global__ void __launch_bounds__(kBlock) k_tile_raster(const int16_t *splats, const uint8_t *lod,
+                                                        uint32_t lod_log2, uint64_t n, SplatCam c,
+                                                        const float4 *rays, uint64_t *frame,
+                                                        const uint64_t *d_n, TileBufs tb)
+{
+    __shared__ float4 lray[kTilePx];
+    __shared__ uint64_t lkey[kTilePx];
+    if (!tb.tc[kTcOk]) return;
+    const uint32_t nitems = tb.tc[kTcItems];
+    if (d_n) n = min(*d_n, n);
+    for (uint32_t it = blockIdx.x; it < nitems; it += gridDim.x) {
+        const uint4 item = tb.items[it];
+        const uint32_t t = item.x & ~kSingle;
+        const int px0 = (int)(t % tb.tx) << kTileLog, py0 = (int)(t / tb.tx) << kTileLog;
+        uint64_t start[kTilePx / kBlock];
+#pragma unroll
+        for (int k = 0; k < kTilePx / kBlock; k++) {
+            const int q = (int)threadIdx.x + k * kBlock;
+            const int px = px0 + (q & (kTile - 1)), py = py0 + (q >> kTileLog);
+            start[k] = ~0ull;
+            if (px < (int)c.W && py < (int)c.H) {
+                lray[q] = rays[(size_t)py * c.W + px];
+                start[k] = frame[(size_t)py * c.W + px];
+            }
+            lkey[q] = start[k];
+        }
+        __syncthreads();
+        for (uint32_t j0 = item.y; j0 < item.z; j0 += blockDim.x) {
+            const uint32_t j = j0 + threadIdx.x;
+            const uint64_t i = j < item.z ? (uint64_t)tb.pairs[j] : n;
+            tile_splat(splats, lod, lod_log2, n, i, c, px0, py0, lray, lkey);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < kTilePx / kBlock; k++) {
+            const int q = (int)threadIdx.x + k * kBlock;
+            const int px = px0 + (q & (kTile - 1)), py = py0 + (q >> kTileLog);
+            if (px >= (int)c.W || py >= (int)c.H) continue;
+            const uint64_t key = lkey[q];
+            if (key >= start[k]) continue;
+            uint64_t *f = frame + (size_t)py * c.W + px;
+            if (item.x & kSingle)
+                *f = key;
+            else
+                atomicMin(reinterpret_cast<unsigned long long *>(f), key);
+        }
+        __syncthreads();
+    }
+}
+
+// The big list against the global frame, before the tiles.
+__global__ void __launch_bounds__(kBlock) k_tile_big(const int16_t *splats, const uint8_t *lod,
+                                                     uint32_t lod_log2, uint64_t n, SplatCam c,
+                                                     const float4 *rays, uint64_t *frame,
+                                                     const uint64_t *d_n, TileBufs tb)
+{
+    if (!tb.tc[kTcOk]) return;
+    const uint32_t nbig = min(tb.tc[kTcBigCur], tb.big_cap);
+    if (d_n) n = min(*d_n, n);
+    for (uint32_t j0 = blockIdx.x * blockDim.x; j0 < nbig; j0 += gridDim.x * blockDim.x) {
+        const uint32_t j = j0 + threadIdx.x;
+        const uint64_t i = j < nbig ? (uint64_t)tb.big[j] : n;
         raster_splat(splats, lod, lod_log2, n, i, c, rays, frame);
     }
 }
@@ -565,8 +944,16 @@ static int splat_frame(vp_ctx *ctx, const int16_t *d_splats, const uint8_t *d_lo
     float4 *rays;
     uint32_t *blk = nullptr;
     int ncu;
+    // screen tiles (block-culled frames of at most kMaxTiles tiles)
+    TileBufs tb = {};
+    tb.tx = (cam->width + kTile - 1) / kTile;
+    tb.ty = (cam->height + kTile - 1) / kTile;
+    tb.nt = tb.tx * tb.ty;
+    uint32_t tgrid = 0;
+    bool tiles = false;
     {
         std::lock_guard<std::mutex> g(ctx->mu);
+        tiles = ctx->splat_tiles && d_bounds && nb && tb.nt <= kMaxTiles;
         int rc = ensure_frame(ctx, np);
         if (rc) return rc;
         if (d_bounds && nb) {
@@ -577,6 +964,33 @@ static int splat_frame(vp_ctx *ctx, const int16_t *d_splats, const uint8_t *d_lo
         frame = ctx->d_frame;
         rays = ctx->d_rays;
         ncu = ctx->num_cus;
+        if (tiles) {
+            // entries: at most four per splat; beyond 2^26 in one frame the
+            // global path draws it
+            tgrid = (uint32_t)ncu * 4u;
+            const uint64_t lim = ctx->splat_tile_entries ? ctx->splat_tile_entries : (1ull << 26);
+            tb.pair_cap = (uint32_t)(4 * n < lim ? 4 * n : lim);
+            tb.big_cap = (uint32_t)(n < (1ull << 22) ? n : (1ull << 22));
+            tb.item_cap = tb.nt + tb.pair_cap / kItem + 1;
+            const uint64_t words = (uint64_t)tgrid * tb.nt + 2ull * tb.nt + kTcWords +
+                                   4ull * tb.item_cap + tb.pair_cap + tb.big_cap;
+            rc = ensure_tiles(ctx, words);
+            if (rc) return rc;
+            uint32_t *w = ctx->d_tiles;
+            tb.items = reinterpret_cast<uint4 *>(w);            // 16-byte aligned
+            w += 4ull * tb.item_cap;
+            tb.hist = w;
+            w += (uint64_t)tgrid * tb.nt;
+            tb.cnt = w;
+            w += tb.nt;
+            tb.cur = w;
+            w += tb.nt;
+            tb.tc = w;
+            w += kTcWords;
+            tb.pairs = w;
+            w += tb.pair_cap;
+            tb.big = w;
+        }
     }
     const SplatCam c = to_cam(cam);
     const dim3 pg((cam->width + 255) / 256, cam->height);
@@ -596,8 +1010,23 @@ static int splat_frame(vp_ctx *ctx, const int16_t *d_splats, const uint8_t *d_lo
                            bin, cnt, d_n);
         hipLaunchKernelGGL(k_blk_scan, dim3(1), dim3(kBins), 0, st, cnt);
         hipLaunchKernelGGL(k_blk_scatter, dim3(g), dim3(256), 0, st, bin, nb, cnt, order, d_n);
+        if (tiles) {
+            const size_t lds = (size_t)tb.nt * sizeof(uint32_t);
+            hipLaunchKernelGGL(k_tile_reset, dim3((tb.nt + 255) / 256), dim3(256), 0, st, tb.cnt,
+                               tb.nt, tb.tc);
+            hipLaunchKernelGGL(k_tile_count, dim3(tgrid), dim3(kBlock), lds, st, d_splats, d_lod,
+                               lod_log2, n, c, order, cnt, d_n, tb);
+            hipLaunchKernelGGL(k_tile_scan, dim3(1), dim3(kScanThreads), 0, st, tb);
+            hipLaunchKernelGGL(k_tile_scatter, dim3(tgrid), dim3(kBlock), lds, st, d_splats, d_lod,
+                               lod_log2, n, c, order, cnt, d_n, tb);
+            hipLaunchKernelGGL(k_tile_big, dim3((uint32_t)ncu * 4u), dim3(kBlock), 0, st, d_splats,
+                               d_lod, lod_log2, n, c, rays, frame, d_n, tb);
+            hipLaunchKernelGGL(k_tile_raster, dim3((uint32_t)ncu * 6u), dim3(kBlock), 0, st,
+                               d_splats, d_lod, lod_log2, n, c, rays, frame, d_n, tb);
+        }
         hipLaunchKernelGGL(k_raster_blocks, dim3((uint32_t)ncu * 8u), dim3(kBlock), 0, st,
-                           d_splats, d_lod, lod_log2, n, c, rays, frame, order, cnt, d_n);
+                           d_splats, d_lod, lod_log2, n, c, rays, frame, order, cnt, d_n,
+                           tiles ? tb.tc + kTcOk : nullptr);
     } else if (n) {
         hipLaunchKernelGGL(k_raster, dim3((uint32_t)nb), dim3(256), 0, st, d_splats, d_lod,
                            lod_log2, n, c, rays, frame);
@@ -638,6 +1067,15 @@ extern "C" int vp_gpu_splat_blocks(vp_ctx *ctx, const int16_t *d_splats, const u
                                    const vp_frame *out, void *stream)
 {
     return splat_frame(ctx, d_splats, d_lod, lod_log2, n, d_bounds, cam, d_voxel_rgb, out, stream);
+}
+
+extern "C" int vp_ctx_set_splat_tiles(vp_ctx *ctx, int enable, uint64_t max_entries)
+{
+    if (!ctx || max_entries > 0xFFFFFFFFull) return VP_ERR_ARG;
+    std::lock_guard<std::mutex> g(ctx->mu);
+    ctx->splat_tiles = enable ? 1 : 0;
+    ctx->splat_tile_entries = max_entries;
+    return VP_OK;
 }
 
 extern "C" int vp_gpu_splat_backward(vp_ctx *ctx, uint64_t n, const vp_camera *cam,

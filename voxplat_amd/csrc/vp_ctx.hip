@@ -68,6 +68,23 @@ int ensure_blocks(vp_ctx *ctx, uint64_t blocks)
     return VP_OK;
 }
 
+int ensure_tiles(vp_ctx *ctx, uint64_t words)
+{
+    if (words <= ctx->tiles_words) return VP_OK;
+    uint32_t *p = nullptr;
+    if (hipMalloc(&p, words * sizeof(uint32_t)) != hipSuccess) return VP_ERR_NOMEM;
+    if (ctx->d_tiles) {
+        if (hipDeviceSynchronize() != hipSuccess) {
+            (void)hipFree(p);
+            return VP_ERR_HIP;
+        }
+        (void)hipFree(ctx->d_tiles);
+    }
+    ctx->d_tiles = p;
+    ctx->tiles_words = words;
+    return VP_OK;
+}
+
 int ensure_cmap(vp_ctx *ctx, uint64_t chunks)
 {
     if (chunks <= ctx->cmap_cap) return VP_OK;
@@ -161,6 +178,7 @@ extern "C" int vp_ctx_destroy(vp_ctx *ctx)
     if (ctx->d_frame) (void)hipFree(ctx->d_frame);
     if (ctx->d_rays) (void)hipFree(ctx->d_rays);
     if (ctx->d_blk) (void)hipFree(ctx->d_blk);
+    if (ctx->d_tiles) (void)hipFree(ctx->d_tiles);
     if (ctx->d_cmap) (void)hipFree(ctx->d_cmap);
     if (ctx->d_oct) (void)hipFree(ctx->d_oct);
     if (ctx->d_oct_part) (void)hipFree(ctx->d_oct_part);
@@ -215,15 +233,56 @@ extern "C" int vp_ctx_set_launch(vp_ctx *ctx, int block, int wg_per_cu)
     if (!ctx || (block != 256 && block != 512 && block != 1024) || wg_per_cu < 1 || wg_per_cu > 4)
         return VP_ERR_ARG;
     std::lock_guard<std::mutex> g(ctx->mu);
-    ctx->cull_block = block;
-    ctx->cull_wg_per_cu = wg_per_cu;
+    ctx->cull_shape.block = block;
+    ctx->cull_shape.wg_per_cu = wg_per_cu;
     return VP_OK;
 }
 
 extern "C" int vp_ctx_get_launch(vp_ctx *ctx, int *block, int *wg_per_cu)
 {
     if (!ctx || !block || !wg_per_cu) return VP_ERR_ARG;
-    *block = ctx->cull_block;
-    *wg_per_cu = ctx->cull_wg_per_cu;
+    std::lock_guard<std::mutex> g(ctx->mu);
+    *block = ctx->cull_shape.block;
+    *wg_per_cu = ctx->cull_shape.wg_per_cu;
+    return VP_OK;
+}
+
+extern "C" int vp_ctx_set_remesh_shape(vp_ctx *ctx, const vp_world_desc *world,
+                                       const vp_remesh_shape *shape)
+{
+    if (!ctx || !shape) return VP_ERR_ARG;
+    const int b = shape->block, w = shape->wg_per_cu, u = shape->records_in_flight;
+    if ((b != 256 && b != 512 && b != 1024) || w < 1 || w > 4 || (u != 2 && u != 4))
+        return VP_ERR_ARG;
+    vp_remesh_shape_i s;
+    s.block = b;
+    s.wg_per_cu = w;
+    s.u = u;
+    std::lock_guard<std::mutex> g(ctx->mu);
+    if (!world) {
+        ctx->cull_shape = s;
+        return VP_OK;
+    }
+    const uint32_t k = vp::world_key(world);
+    for (int i = 0; i < ctx->n_cull_shapes; i++)
+        if (ctx->cull_shape_key[i] == k) {
+            ctx->cull_shapes[i] = s;
+            return VP_OK;
+        }
+    if (ctx->n_cull_shapes == vp_ctx::kMaxShapes) return VP_ERR_CAPACITY;
+    ctx->cull_shape_key[ctx->n_cull_shapes] = k;
+    ctx->cull_shapes[ctx->n_cull_shapes++] = s;
+    return VP_OK;
+}
+
+extern "C" int vp_ctx_get_remesh_shape(vp_ctx *ctx, const vp_world_desc *world,
+                                       vp_remesh_shape *shape)
+{
+    if (!ctx || !shape) return VP_ERR_ARG;
+    std::lock_guard<std::mutex> g(ctx->mu);
+    const vp_remesh_shape_i s = world ? vp::remesh_shape(ctx, world) : ctx->cull_shape;
+    shape->block = s.block;
+    shape->wg_per_cu = s.wg_per_cu;
+    shape->records_in_flight = s.u;
     return VP_OK;
 }

@@ -13,6 +13,15 @@
         if (e_ != hipSuccess) return VP_ERR_HIP;           \
     } while (0)
 
+// Launch shape of the persistent remesh kernel: threads per workgroup,
+// workgroups per CU the grid is sized for, records each lane prepares
+// before its stores.
+struct vp_remesh_shape_i {
+    int block = 512;
+    int wg_per_cu = 2;
+    int u = 4;
+};
+
 struct vp_ctx {
     int device = 0;
     int num_cus = 0;
@@ -30,6 +39,10 @@ struct vp_ctx {
     uint32_t *d_blk = nullptr;         // splat block culling: bins, order, counters
     uint64_t  blk_cap = 0;             // blocks d_blk is sized for
     uint64_t  rays_cap = 0;
+    uint32_t *d_tiles = nullptr;       // splat screen tiles: histograms, counts, items, entries
+    uint64_t  tiles_words = 0;         // u32 words of d_tiles
+    int       splat_tiles = 0;         // resolve block-culled frames on screen tiles
+    uint64_t  splat_tile_entries = 0;  // entry capacity of a tiled frame (0: default)
     uint32_t *d_cmap = nullptr;        // draw lists: list entry + 1 per chunk index
     uint64_t  cmap_cap = 0;
     uint32_t *d_oct = nullptr;         // device draw-queue selection: key slots
@@ -39,9 +52,13 @@ struct vp_ctx {
     // device address of the static scratch of vp_gpu_cull_mesh_noalloc on
     // this context's device (noalloc.hip); not owned
     uint64_t *d_noalloc = nullptr;
-    // tuned launch shape of the persistent remesh kernel
-    int cull_block = 512;              // threads per workgroup
-    int cull_wg_per_cu = 2;            // resident workgroups per CU
+    // launch shape of the persistent remesh kernel: a default, and tuned
+    // shapes per world geometry (key: root_bitw | max_bitw[a] << 8 (a + 1))
+    vp_remesh_shape_i cull_shape;
+    static constexpr int kMaxShapes = 16;
+    uint32_t cull_shape_key[kMaxShapes] = {};
+    vp_remesh_shape_i cull_shapes[kMaxShapes];
+    int n_cull_shapes = 0;
 };
 
 namespace vp {
@@ -50,11 +67,29 @@ int ensure_state(vp_ctx *ctx, uint64_t words);
 int ensure_frame(vp_ctx *ctx, uint64_t pixels);
 int ensure_blocks(vp_ctx *ctx, uint64_t blocks);
 int ensure_cmap(vp_ctx *ctx, uint64_t chunks);
+int ensure_tiles(vp_ctx *ctx, uint64_t words);
 int ensure_oct(vp_ctx *ctx, uint64_t slots, uint64_t blocks);
 // fetch the static scratch's device address on the current device
 int noalloc_init(vp_ctx *ctx);
 
 inline hipStream_t as_stream(void *s) { return reinterpret_cast<hipStream_t>(s); }
+
+inline uint32_t world_key(const vp_world_desc *w)
+{
+    return w->root_bitw | (uint32_t)w->max_bitw[0] << 8 | (uint32_t)w->max_bitw[1] << 16 |
+           (uint32_t)w->max_bitw[2] << 24;
+}
+
+// The remesh launch shape for a world: its tuned shape, else the default.
+// Read without the context lock (launches may hold it); shapes are set
+// between calls, not during them.
+inline vp_remesh_shape_i remesh_shape(const vp_ctx *ctx, const vp_world_desc *w)
+{
+    const uint32_t k = world_key(w);
+    for (int i = 0; i < ctx->n_cull_shapes; i++)
+        if (ctx->cull_shape_key[i] == k) return ctx->cull_shapes[i];
+    return ctx->cull_shape;
+}
 
 // vp_gpu_cull_mesh's argument checks, and its launch with explicit scratch:
 // a ticket word and n look-back words (both unused when n == 1 and d_n is
@@ -74,6 +109,21 @@ int rle_encode_launch(vp_ctx *ctx, const vp_world_desc *world, const uint8_t *d_
                       const uint32_t *d_chunk_ids, uint32_t n, const uint32_t *d_n,
                       uint32_t *d_words, uint64_t capacity_words, uint64_t *d_word_offsets,
                       int16_t *d_uniform_out, uint32_t *ticket, uint64_t *state, hipStream_t st);
+
+// Stores of output the kernel never reads back: streamed past the caches
+// (non-temporal), so they do not evict the inputs still being read.
+__device__ __forceinline__ void nt_store16(void *p, uint4 v)
+{
+    typedef unsigned int v4u __attribute__((ext_vector_type(4)));
+    const v4u x = {v.x, v.y, v.z, v.w};
+    __builtin_nontemporal_store(x, reinterpret_cast<v4u *>(p));
+}
+__device__ __forceinline__ void nt_store8(void *p, uint2 v)
+{
+    typedef unsigned int v2u __attribute__((ext_vector_type(2)));
+    const v2u x = {v.x, v.y};
+    __builtin_nontemporal_store(x, reinterpret_cast<v2u *>(p));
+}
 
 // relaxed device-scope atomics on 64-bit words (coherent across XCDs)
 __device__ __forceinline__ uint64_t ld_relaxed(const uint64_t *p)

@@ -111,11 +111,16 @@ int vp_host_autotune_remesh(vp_ctx *ctx, const vp_world_desc *world,
                             const uint8_t *d_voxels, const int16_t *d_uniform,
                             const vp_shadow_desc *shadow, const uint32_t *d_chunk_ids,
                             uint32_t n, uint32_t flags, const vp_splat_out *out, int reps,
-                            void *stream, int *best_block, int *best_wg_per_cu,
-                            float *best_ms)
+                            void *stream, vp_remesh_shape *best, float *best_ms,
+                            int *shapes_timed)
 {
     static const int blocks[3] = {256, 512, 1024};
-    if (!ctx || !best_block || !best_wg_per_cu || reps < 1 || reps > 64) return VP_ERR_ARG;
+    if (!ctx || !world || !best || reps < 1 || reps > 64) return VP_ERR_ARG;
+    /* the shape the calls below run with is the world's own: remember the
+     * one in force, to restore it if a call fails */
+    vp_remesh_shape keep;
+    int rc = vp_ctx_get_remesh_shape(ctx, world, &keep);
+    if (rc) return rc;
     hipStream_t st = (hipStream_t)stream;
     hipEvent_t e0, e1;
     if (hipEventCreate(&e0) != hipSuccess) return VP_ERR_HIP;
@@ -123,38 +128,52 @@ int vp_host_autotune_remesh(vp_ctx *ctx, const vp_world_desc *world,
         (void)hipEventDestroy(e0);
         return VP_ERR_HIP;
     }
-    float best = 1e30f, t[64];
-    int bb = 512, bw = 2, rc = VP_OK;
+    const int nu = world->root_bitw == 6 ? 2 : 1;
+    float bestt = 1e30f, t[64];
+    vp_remesh_shape bs = keep;
+    int timed = 0;
     for (int bi = 0; bi < 3 && rc == VP_OK; bi++)
-        for (int wg = 1; wg <= 4 && rc == VP_OK; wg++) {
-            if (blocks[bi] * wg > 2048) continue;          /* 32 waves per CU at most */
-            rc = vp_ctx_set_launch(ctx, blocks[bi], wg);
+        for (int ui = 0; ui < nu && rc == VP_OK; ui++) {
+            vp_remesh_shape s = {blocks[bi], 1, ui ? 2 : 4};
+            int resident = 0;
+            rc = vp_ctx_remesh_resident(ctx, world, flags, &s, &resident);
             if (rc) break;
-            rc = vp_gpu_cull_mesh(ctx, world, d_voxels, d_uniform, shadow, d_chunk_ids, n, NULL,
-                                  flags, out, stream);   /* warm-up */
-            for (int r = 0; r < reps && rc == VP_OK; r++) {
-                if (hipEventRecord(e0, st) != hipSuccess) { rc = VP_ERR_HIP; break; }
+            if (resident > 4) resident = 4;
+            for (int wg = 1; wg <= resident && rc == VP_OK; wg++) {
+                s.wg_per_cu = wg;
+                rc = vp_ctx_set_remesh_shape(ctx, world, &s);
+                if (rc) break;
                 rc = vp_gpu_cull_mesh(ctx, world, d_voxels, d_uniform, shadow, d_chunk_ids, n,
-                                      NULL, flags, out, stream);
-                if (hipEventRecord(e1, st) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
-                    hipEventElapsedTime(&t[r], e0, e1) != hipSuccess)
-                    rc = VP_ERR_HIP;
-            }
-            if (rc) break;
-            qsort(t, (size_t)reps, sizeof(float), cmp_float);
-            if (t[reps / 2] < best) {
-                best = t[reps / 2];
-                bb = blocks[bi];
-                bw = wg;
+                                      NULL, flags, out, stream);   /* warm-up */
+                for (int r = 0; r < reps && rc == VP_OK; r++) {
+                    if (hipEventRecord(e0, st) != hipSuccess) { rc = VP_ERR_HIP; break; }
+                    rc = vp_gpu_cull_mesh(ctx, world, d_voxels, d_uniform, shadow, d_chunk_ids,
+                                          n, NULL, flags, out, stream);
+                    if (hipEventRecord(e1, st) != hipSuccess ||
+                        hipEventSynchronize(e1) != hipSuccess ||
+                        hipEventElapsedTime(&t[r], e0, e1) != hipSuccess)
+                        rc = VP_ERR_HIP;
+                }
+                if (rc) break;
+                qsort(t, (size_t)reps, sizeof(float), cmp_float);
+                timed++;
+                if (t[reps / 2] < bestt) {
+                    bestt = t[reps / 2];
+                    bs = s;
+                }
             }
         }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    if (rc) return rc;
-    *best_block = bb;
-    *best_wg_per_cu = bw;
-    if (best_ms) *best_ms = best;
-    return vp_ctx_set_launch(ctx, bb, bw);
+    if (rc) {
+        (void)vp_ctx_set_remesh_shape(ctx, world, &keep);
+        return rc;
+    }
+    if (timed == 0) return VP_ERR_UNSUPPORTED;
+    *best = bs;
+    if (best_ms) *best_ms = bestt;
+    if (shapes_timed) *shapes_timed = timed;
+    return vp_ctx_set_remesh_shape(ctx, world, &bs);
 }
 
 /* ---- frame draw queue -------------------------------------------------------

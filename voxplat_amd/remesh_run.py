@@ -64,15 +64,17 @@ def main(argv=None) -> int:
     cap = int(offs[-1].item()) // 4
     splats = torch.empty((max(cap, 1), 4), dtype=torch.int16, device="cuda")
     if a.autotune:   # the launch shape the benchmark would pick
-        b, wg, _ = gpu.autotune_remesh(w, vox, ids, smap, splats, cap, offs, counts, uniform=uni,
-                                       reps=3)
-        print(f"launch shape: {b} threads x {wg} per CU", file=sys.stderr, flush=True)
+        t = gpu.autotune_remesh(w, vox, ids, smap, splats, cap, offs, counts, uniform=uni,
+                                reps=3)
+        print(f"launch shape: {t['block']} threads x {t['wg_per_cu']} per CU, "
+              f"{t['records_in_flight']} records in flight ({t['shapes_timed']} shapes timed)",
+              file=sys.stderr, flush=True)
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     ms = []
     for _ in range(a.launches):
         e0.record()
         gpu.cull_mesh(w, vox, ids, shadow_map=smap, uniform=uni, capacity=cap,
-                      out=(offs, counts))
+                      out=(offs, counts), records=splats)
         e1.record()
         e1.synchronize()
         ms.append(e0.elapsed_time(e1))

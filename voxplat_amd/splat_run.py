@@ -1,12 +1,15 @@
 """Splat resolve of the 2048x256x2048 level-0 list at 1920x1080, for profilers.
 
-    python -m voxplat_amd.splat_run [--poses P] [--rgb]
+    python -m voxplat_amd.splat_run [--poses P] [--rgb] [--plain]
+                                    [--path tiles|global] [--check]
 
 Generates the world on the GPU, remeshes it (level 0 only, chunk-index
 order: 130.5 M splats), computes the list's block bounds once, then renders
 the seeded poses one after another with vp_gpu_splat_blocks (--plain:
 vp_gpu_splat), one synchronised frame each, and prints the per-frame times
-measured with HIP events.
+measured with HIP events.  --path picks the screen-tile resolve or the
+global per-fragment path (default); --check then draws every pose again by
+the other path and compares the frames bit for bit.
 """
 from __future__ import annotations
 
@@ -54,8 +57,12 @@ def main(argv=None) -> int:
     ap.add_argument("--rgb", action="store_true", help="also shade RGB")
     ap.add_argument("--plain", action="store_true",
                     help="vp_gpu_splat without block bounds (no culling or ordering)")
+    ap.add_argument("--path", choices=("tiles", "global"), default="global")
+    ap.add_argument("--check", action="store_true",
+                    help="also draw every pose by the other path and compare the frames")
     a = ap.parse_args(argv)
     gpu = GPU(0)
+    gpu.set_splat_tiles(a.path == "tiles")
     w, vox, l0 = level0_list(gpu)
     cams = poses_for(w, vox, a.poses, 1920, 1080)
     gpu.reserve(0, 1920 * 1080)
@@ -75,9 +82,22 @@ def main(argv=None) -> int:
         e1.synchronize()
         ms.append(e0.elapsed_time(e1))
         hit.append(float((ids != -1).float().mean()))
-    print(json.dumps({"splats": int(l0.shape[0]), "frame_ms": [round(x, 4) for x in ms],
+    same = None
+    if a.check:
+        same = True
+        for cam in cams:
+            gpu.set_splat_tiles(a.path == "tiles")
+            gpu.splat(l0, cam, out=out, rgb=a.rgb, bounds=bounds)
+            d0, i0 = depth.clone(), ids.clone()
+            gpu.set_splat_tiles(a.path != "tiles")
+            gpu.splat(l0, cam, out=out, rgb=a.rgb, bounds=bounds)
+            same = same and bool(torch.equal(i0, ids)) and bool(torch.equal(d0, depth))
+        gpu.set_splat_tiles(a.path == "tiles")
+    print(json.dumps({"splats": int(l0.shape[0]), "path": a.path,
+                      "frame_ms": [round(x, 4) for x in ms],
                       "mean_ms": round(sum(ms) / len(ms), 4),
-                      "pixels_hit": round(sum(hit) / len(hit), 4), "status": gpu.status()}))
+                      "pixels_hit": round(sum(hit) / len(hit), 4),
+                      "paths_identical": same, "status": gpu.status()}))
     gpu.close()
     return 0
 

@@ -205,10 +205,13 @@ def rle_bytes(n_chunks: int, words: int, R3: int = 262144) -> int:
     return n_chunks * (R3 + 10) + 4 * words
 
 
-def full_remesh(world: ResidentWorld, steps: int = 3, warmup: int = 1, log=None):
+def full_remesh(world: ResidentWorld, steps: int = 3, warmup: int = 1, log=None,
+                autotune: bool = False):
     """Remesh every chunk of the world (L0-L4) as one graph launch per step,
-    the list ordered heaviest first.  Returns timings; the record buffer is
-    freed before returning."""
+    the list ordered heaviest first; with `autotune`, the launch shape is
+    first tuned on this list and kept as the world's shape (the frames' graph
+    then uses it too).  Returns timings; the record buffer is freed before
+    returning."""
     gpu, w = world.gpu, world.w
     ids = torch.arange(w.chunks, dtype=torch.int32, device=world.voxels.device)
     ids, offs, counts = heaviest_first(gpu, w, world.voxels, ids, world.smap, world.uniform)
@@ -218,6 +221,13 @@ def full_remesh(world: ResidentWorld, steps: int = 3, warmup: int = 1, log=None)
         log(f"full remesh: {records} records ({records * 8 / 2**30:.1f} GiB)")
     splats = torch.empty((max(records, 1), 4), dtype=torch.int16, device=world.voxels.device)
     gpu.reserve(w.chunks, 0)
+    tune = None
+    if autotune:
+        tune = gpu.autotune_remesh(w, world.voxels, ids, world.smap, splats, records, offs,
+                                   counts, uniform=world.uniform, reps=1)
+        tune["ms"] = round(tune["ms"], 3)
+        if log:
+            log(f"launch shape: {tune}")
     g = RemeshGraph(gpu, w, world.voxels, ids, world.smap, offs, counts, splats, records,
                     uniform=world.uniform)
     st = torch.cuda.current_stream()
@@ -244,7 +254,7 @@ def full_remesh(world: ResidentWorld, steps: int = 3, warmup: int = 1, log=None)
             "kernel_ms": round(ms, 3),
             "gvoxels_s": round(w.chunks * world.R3 / (max(ms, wall) * 1e-3) / 1e9, 2),
             "algorithmic_bytes": algo, "gb_s": round(algo / (ms * 1e-3) / 1e9, 1),
-            "status": status}
+            "launch": tune if tune else gpu.get_remesh_shape(w), "status": status}
 
 
 def main(argv=None) -> int:
@@ -264,7 +274,8 @@ def main(argv=None) -> int:
 
 def run_world16k(frames: int = 4, edits: int = 10000, log=None, keep: bool = False,
                  gpu: GPU | None = None, full_steps: int = 0, full_warmup: int = 1,
-                 on_generated=None, record_capacity: int = 1 << 31, word_margin: float = 1.25):
+                 on_generated=None, record_capacity: int = 1 << 31, word_margin: float = 1.25,
+                 autotune: bool = False):
     """The 16384x512x16384 workload.  Each frame is a new burst of `edits`
     seeded sphere edits (seed EDIT_SEED + frame), then one replay of the frame
     graph: lists, remesh of the dirty chunks, RLE of the others.  Returns a
@@ -287,7 +298,8 @@ def run_world16k(frames: int = 4, edits: int = 10000, log=None, keep: bool = Fal
     occ = {"uniform_air": int((uni_h == 0).sum()), "uniform_solid": int((uni_h > 0).sum()),
            "mixed": int((uni_h < 0).sum())}
     say(f"generated {cfg.name} in {t_gen:.2f} s, occupancy {occ}")
-    full = full_remesh(world, steps=full_steps, warmup=full_warmup, log=say) if full_steps else None
+    full = full_remesh(world, steps=full_steps, warmup=full_warmup, log=say,
+                       autotune=autotune) if full_steps else None
     if full:
         say(f"full remesh: {full}")
     extra = on_generated(world) if on_generated else None
