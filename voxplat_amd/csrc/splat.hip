@@ -147,12 +147,16 @@ __global__ void k_clear(uint64_t *frame, uint64_t np)
         frame[i] = ~0ull;
 }
 
-__device__ __forceinline__ void fragment(const SplatCam &c, const float4 *rays, uint64_t *frame,
-                                         float vx, float vy, float vz, float half, uint32_t id,
-                                         uint64_t lb, int px, int py)
+// The fragment of pixel p whose frame key was read as cur: GL_LESS on
+// (depth bits << 32 | id) by one atomicMin.  cur may be older than the
+// frame (the loads of a batch go out before any of its tests): a key at or
+// above an older value is at or above the current one too, so the tests
+// only ever skip fragments that could not win.
+__device__ __forceinline__ void fragment_test(const SplatCam &c, const float4 *rays,
+                                              uint64_t *frame, float vx, float vy, float vz,
+                                              float half, uint32_t id, uint64_t lb, uint32_t p,
+                                              uint64_t cur)
 {
-    const size_t p = (size_t)py * c.W + (uint32_t)px;
-    const uint64_t cur = frame[p];
     if (lb > cur) return;          // every key of this splat is above the stored one
     const float4 inv = rays[p];
     float t0, t1;
@@ -165,11 +169,49 @@ __device__ __forceinline__ void fragment(const SplatCam &c, const float4 *rays, 
     if (key < cur) atomicMin(reinterpret_cast<unsigned long long *>(frame + p), key);
 }
 
-constexpr int kSmallArea = 32;   // 16 and 48 measured 5% slower at 1080p
+// Sprites of up to kSmallArea pixels are rasterised by their own lane,
+// larger ones by the whole wave, a few at a time.  Most fragments end at
+// the depth lower bound after one frame load, so the loads are issued in
+// batches (kLoads per lane) before any test: the lane waits once per batch
+// instead of once per pixel.
+constexpr int kSmallArea = 64;
+constexpr int kLoads = 8;
+constexpr int kGroup = 4;                 // wave-shared sprites per round
+constexpr int kSteps = kLoads / kGroup;   // pixels of each per lane per round
+
+// Lane src's value (src wave-uniform) through a scalar register: no LDS
+// traffic, unlike a general shuffle.
+__device__ __forceinline__ int lane_i(int v, int src) { return __builtin_amdgcn_readlane(v, src); }
+__device__ __forceinline__ float lane_f(float v, int src)
+{
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), src));
+}
+
+// A wave-shared sprite: the owning lane's vertex stage, broadcast.
+struct WaveSprite {
+    float vx, vy, vz, half, lox, hix, loy, hiy;
+    int x0, y0, w, a, dq, dr;
+    uint32_t id;
+    uint64_t lb;
+};
+
+__device__ __forceinline__ WaveSprite wave_sprite(const VS &vs, int area, uint32_t id, int src)
+{
+    WaveSprite s;
+    s.vx = lane_f(vs.vx, src); s.vy = lane_f(vs.vy, src);
+    s.vz = lane_f(vs.vz, src); s.half = lane_f(vs.half, src);
+    s.lox = lane_f(vs.lox, src); s.hix = lane_f(vs.hix, src);
+    s.loy = lane_f(vs.loy, src); s.hiy = lane_f(vs.hiy, src);
+    s.x0 = lane_i(vs.x0, src);
+    s.y0 = lane_i(vs.y0, src);
+    s.w = lane_i(vs.x1, src) - s.x0 + 1;
+    s.a = lane_i(area, src);
+    s.id = (uint32_t)lane_i((int)id, src);
+    s.lb = (uint64_t)(uint32_t)lane_i((int)(uint32_t)(vs.lb >> 32), src) << 32;
+    return s;
+}
 
 // Splat i (i >= n: none) of one lane; every lane of the wave calls it.
-// Sprites larger than kSmallArea pixels are shared by the whole wave, one at
-// a time, so a few near splats do not stall a wave.
 __device__ __forceinline__ void raster_splat(const int16_t *splats, const uint8_t *lod,
                                              uint32_t lod_log2, uint64_t n, uint64_t i,
                                              const SplatCam &c, const float4 *rays,
@@ -185,47 +227,91 @@ __device__ __forceinline__ void raster_splat(const int16_t *splats, const uint8_
     if (ok && vs.x0 <= vs.x1 && vs.y0 <= vs.y1) area = (vs.x1 - vs.x0 + 1) * (vs.y1 - vs.y0 + 1);
     const uint32_t id = (uint32_t)i;
     if (area > 0 && area <= kSmallArea) {
-        for (int py = vs.y0; py <= vs.y1; py++) {
-            const float cy = (float)py + 0.5f;
-            if (!(vs.loy <= cy && cy < vs.hiy)) continue;
-            for (int px = vs.x0; px <= vs.x1; px++) {
-                const float cx = (float)px + 0.5f;
-                if (!(vs.lox <= cx && cx < vs.hix)) continue;
-                fragment(c, rays, frame, vs.vx, vs.vy, vs.vz, vs.half, id, vs.lb, px, py);
+        // the box's pixels row by row, kLoads at a time
+        const int w = vs.x1 - vs.x0 + 1;
+        int col = 0, row = 0;
+        for (int k0 = 0; k0 < area; k0 += kLoads) {
+            uint32_t pp[kLoads];
+            uint64_t cur[kLoads];
+            uint32_t live = 0;
+#pragma unroll
+            for (int u = 0; u < kLoads; u++) {
+                const int px = vs.x0 + col, py = vs.y0 + row;
+                const float cx = (float)px + 0.5f, cy = (float)py + 0.5f;
+                pp[u] = (uint32_t)py * c.W + (uint32_t)px;
+                cur[u] = 0;
+                if (k0 + u < area && vs.lox <= cx && cx < vs.hix && vs.loy <= cy && cy < vs.hiy) {
+                    cur[u] = frame[pp[u]];
+                    live |= 1u << u;
+                }
+                if (++col == w) { col = 0; ++row; }
             }
+#pragma unroll
+            for (int u = 0; u < kLoads; u++)
+                if ((live >> u) & 1u)
+                    fragment_test(c, rays, frame, vs.vx, vs.vy, vs.vz, vs.half, id, vs.lb, pp[u],
+                                  cur[u]);
         }
     }
     uint64_t big = __ballot(area > kSmallArea);
     const int lane = (int)(threadIdx.x & 63u);
     while (big) {
-        const int src = __builtin_ctzll(big);
-        big &= big - 1ull;
-        const float vx = __shfl(vs.vx, src, 64), vy = __shfl(vs.vy, src, 64);
-        const float vz = __shfl(vs.vz, src, 64), half = __shfl(vs.half, src, 64);
-        const float lox = __shfl(vs.lox, src, 64), hix = __shfl(vs.hix, src, 64);
-        const float loy = __shfl(vs.loy, src, 64), hiy = __shfl(vs.hiy, src, 64);
-        const int x0 = __shfl(vs.x0, src, 64), x1 = __shfl(vs.x1, src, 64);
-        const int y0 = __shfl(vs.y0, src, 64);
-        const int a = __shfl(area, src, 64);
-        const uint32_t sid = __shfl(id, src, 64);
-        const uint64_t lb = ((uint64_t)(uint32_t)__shfl((int)(uint32_t)(vs.lb >> 32), src, 64)) << 32;
-        // lane k walks pixels k, k+64, ... of the box row by row; the column
-        // and row advance by 64 % w and 64 / w per step.  The quotients come
-        // from an approximate reciprocal: for a < 2^7 the fraction of
-        // (a + 1/2)/w is at least 1/(2w) >= 2^-12 away from an integer, far
-        // above the reciprocal's error, so the floor is exact.
-        const int w = x1 - x0 + 1;
-        const float rw = __builtin_amdgcn_rcpf((float)w);
-        const int dq = (int)(64.5f * rw), dr = 64 - dq * w;
-        int row = (int)(((float)lane + 0.5f) * rw), col = lane - row * w;
-        for (int k = lane; k < a; k += 64) {
-            const int px = x0 + col, py = y0 + row;
-            col += dr;
-            row += dq;
-            if (col >= w) { col -= w; row += 1; }
-            const float cx = (float)px + 0.5f, cy = (float)py + 0.5f;
-            if (!(lox <= cx && cx < hix && loy <= cy && cy < hiy)) continue;
-            fragment(c, rays, frame, vx, vy, vz, half, sid, lb, px, py);
+        // up to kGroup sprites; lane k takes their pixels k, k + 64, ... row
+        // by row, the column and row advancing by 64 % w and 64 / w per step.
+        // The quotients come from an approximate reciprocal: for a < 2^7 the
+        // fraction of (a + 1/2)/w is at least 1/(2w) >= 2^-12 away from an
+        // integer, far above the reciprocal's error, so the floor is exact.
+        WaveSprite sp[kGroup];
+        int row[kGroup], col[kGroup];
+        int amax = 0;
+#pragma unroll
+        for (int g = 0; g < kGroup; g++) {
+            sp[g] = WaveSprite{};
+            sp[g].w = 1;
+            row[g] = col[g] = 0;
+            if (big) {
+                const int src = __builtin_ctzll(big);
+                big &= big - 1ull;
+                sp[g] = wave_sprite(vs, area, id, src);
+                const float rw = __builtin_amdgcn_rcpf((float)sp[g].w);
+                sp[g].dq = (int)(64.5f * rw);
+                sp[g].dr = 64 - sp[g].dq * sp[g].w;
+                row[g] = (int)(((float)lane + 0.5f) * rw);
+                col[g] = lane - row[g] * sp[g].w;
+                amax = sp[g].a > amax ? sp[g].a : amax;
+            }
+        }
+        for (int k0 = 0; k0 < amax; k0 += 64 * kSteps) {
+            uint32_t pp[kLoads];
+            uint64_t cur[kLoads];
+            uint32_t live = 0;
+#pragma unroll
+            for (int g = 0; g < kGroup; g++)
+#pragma unroll
+                for (int j = 0; j < kSteps; j++) {
+                    const int u = g * kSteps + j;
+                    const int px = sp[g].x0 + col[g], py = sp[g].y0 + row[g];
+                    col[g] += sp[g].dr;
+                    row[g] += sp[g].dq;
+                    if (col[g] >= sp[g].w) { col[g] -= sp[g].w; row[g] += 1; }
+                    const float cx = (float)px + 0.5f, cy = (float)py + 0.5f;
+                    pp[u] = (uint32_t)py * c.W + (uint32_t)px;
+                    cur[u] = 0;
+                    if (k0 + 64 * j + lane < sp[g].a && sp[g].lox <= cx && cx < sp[g].hix &&
+                        sp[g].loy <= cy && cy < sp[g].hiy) {
+                        cur[u] = frame[pp[u]];
+                        live |= 1u << u;
+                    }
+                }
+#pragma unroll
+            for (int g = 0; g < kGroup; g++)
+#pragma unroll
+                for (int j = 0; j < kSteps; j++) {
+                    const int u = g * kSteps + j;
+                    if ((live >> u) & 1u)
+                        fragment_test(c, rays, frame, sp[g].vx, sp[g].vy, sp[g].vz, sp[g].half,
+                                      sp[g].id, sp[g].lb, pp[u], cur[u]);
+                }
         }
     }
 }
@@ -243,7 +329,7 @@ __global__ void __launch_bounds__(256) k_raster(const int16_t *splats, const uin
 // Block b is splats [256b, 256b+256); its bounds are the box of the splat
 // centres and their largest half-size.  Blocks wholly outside the view volume
 // are dropped; the rest are bucketed by distance and rasterised nearest bucket
-// first, so the early depth rejection in fragment() sees the occluders first.
+// first, so the early depth rejection in the raster sees the occluders first.
 constexpr int kBlock = VP_SPLAT_BLOCK;
 constexpr int kBins = 256;
 // Rasterised in phases of growing distance: the nearest eighth of the
@@ -633,7 +719,7 @@ __global__ void __launch_bounds__(kBlock) k_tile_scatter(const int16_t *splats, 
             uint32_t base = 0;
             if (lane == (uint32_t)__builtin_ctzll(m))
                 base = atomicAdd(tb.tc + kTcBigCur, (uint32_t)__popcll(m));
-            base = (uint32_t)__shfl((int)base, __builtin_ctzll(m), 64);
+            base = (uint32_t)lane_i((int)base, __builtin_ctzll(m));
             if (k == 2) {
                 const uint32_t q = base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
                 if (q < tb.big_cap) tb.big[q] = (uint32_t)i;
@@ -701,15 +787,15 @@ __device__ __forceinline__ void tile_splat(const int16_t *splats, const uint8_t 
     while (big) {
         const int src = __builtin_ctzll(big);
         big &= big - 1ull;
-        const float vx = __shfl(vs.vx, src, 64), vy = __shfl(vs.vy, src, 64);
-        const float vz = __shfl(vs.vz, src, 64), half = __shfl(vs.half, src, 64);
-        const float lox = __shfl(vs.lox, src, 64), hix = __shfl(vs.hix, src, 64);
-        const float loy = __shfl(vs.loy, src, 64), hiy = __shfl(vs.hiy, src, 64);
-        const int bx0 = __shfl(x0, src, 64), bx1 = __shfl(x1, src, 64);
-        const int by0 = __shfl(y0, src, 64);
-        const int a = __shfl(area, src, 64);
-        const uint32_t sid = __shfl(id, src, 64);
-        const uint64_t lb = ((uint64_t)(uint32_t)__shfl((int)(uint32_t)(vs.lb >> 32), src, 64)) << 32;
+        const float vx = lane_f(vs.vx, src), vy = lane_f(vs.vy, src);
+        const float vz = lane_f(vs.vz, src), half = lane_f(vs.half, src);
+        const float lox = lane_f(vs.lox, src), hix = lane_f(vs.hix, src);
+        const float loy = lane_f(vs.loy, src), hiy = lane_f(vs.hiy, src);
+        const int bx0 = lane_i(x0, src), bx1 = lane_i(x1, src);
+        const int by0 = lane_i(y0, src);
+        const int a = lane_i(area, src);
+        const uint32_t sid = (uint32_t)lane_i((int)id, src);
+        const uint64_t lb = (uint64_t)(uint32_t)lane_i((int)(uint32_t)(vs.lb >> 32), src) << 32;
         // lane k takes pixels k, k + 64, ... of the clipped box, row by row
         // (w <= kTile, so 64 / w and the row of k come exactly from the
         // approximate reciprocal, as in raster_splat)
@@ -1024,7 +1110,8 @@ static int splat_frame(vp_ctx *ctx, const int16_t *d_splats, const uint8_t *d_lo
             hipLaunchKernelGGL(k_tile_raster, dim3((uint32_t)ncu * 6u), dim3(kBlock), 0, st,
                                d_splats, d_lod, lod_log2, n, c, rays, frame, d_n, tb);
         }
-        hipLaunchKernelGGL(k_raster_blocks, dim3((uint32_t)ncu * 8u), dim3(kBlock), 0, st,
+        // 4 workgroups per CU: what the raster's registers leave resident
+        hipLaunchKernelGGL(k_raster_blocks, dim3((uint32_t)ncu * 4u), dim3(kBlock), 0, st,
                            d_splats, d_lod, lod_log2, n, c, rays, frame, order, cnt, d_n,
                            tiles ? tb.tc + kTcOk : nullptr);
     } else if (n) {
