@@ -265,9 +265,12 @@ def main(argv=None) -> int:
     ap.add_argument("--edits", type=int, default=10000)
     ap.add_argument("--full-steps", type=int, default=0,
                     help="also time a remesh of every chunk (needs ~100 GB more HBM)")
+    ap.add_argument("--warmup", type=int, default=1, help="untimed full remeshes first")
+    ap.add_argument("--autotune", action="store_true",
+                    help="tune the launch shape on the full list first (with --full-steps)")
     a = ap.parse_args(argv)
-    res = run_world16k(a.frames, a.edits, full_steps=a.full_steps,
-                       log=lambda m: print(m, file=sys.stderr, flush=True))
+    res = run_world16k(a.frames, a.edits, full_steps=a.full_steps, full_warmup=a.warmup,
+                       autotune=a.autotune, log=lambda m: print(m, file=sys.stderr, flush=True))
     print(json.dumps(res))
     return 0
 
