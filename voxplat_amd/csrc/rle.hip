@@ -329,7 +329,7 @@ __global__ void __launch_bounds__(BS) k_rle_decode(RleArgs a)
                 const uint32_t v4 = b * 0x01010101u;
                 const uint4 q = make_uint4(v4, v4, v4, v4);
                 for (int s = threadIdx.x; s < VOX / 16; s += BS)
-                    *reinterpret_cast<uint4 *>(dst + (size_t)s * 16) = q;
+                    nt_store16(dst + (size_t)s * 16, q);
                 continue;   // uniform across the workgroup
             }
         }
@@ -414,7 +414,7 @@ __global__ void __launch_bounds__(BS) k_rle_decode(RleArgs a)
                 if (ri >= 0 && p0 + j < pos) v = in_lds ? (uint32_t)val8[ri] : (wd[ri] >> 24);
                 out[j >> 2] |= v << ((j & 3) * 8);
             }
-            *reinterpret_cast<uint4 *>(dst + p0) = make_uint4(out[0], out[1], out[2], out[3]);
+            nt_store16(dst + p0, make_uint4(out[0], out[1], out[2], out[3]));
         }
         if (threadIdx.x == 0 && s_bad) atomicOr(a.status, (uint32_t)VP_STATUS_BAD_RLE);
         __syncthreads();
