@@ -31,7 +31,7 @@ def test_generate_and_classify(gpu, cfg):
 
 
 @pytest.mark.parametrize("rb,mb,density,seed", [(2, (2, 2, 2), 0.5, 1), (3, (2, 1, 2), 0.3, 2),
-                                                (6, (2, 2, 1), 0.6, 3)])
+                                                (6, (2, 2, 1), 0.6, 3), (6, (1, 4, 1), 0.4, 4)])
 def test_shadow_build_matches_serial_order(gpu, rb, mb, density, seed):
     import torch
 

@@ -204,6 +204,70 @@ __global__ void __launch_bounds__(256) k_shadow_rows(const uint8_t *voxels, uint
     }
 }
 
+// The same map for 64^3 chunks with every band's serial chain in flight at
+// once: one workgroup per map row, one wave per band.  Lane y loads voxel row
+// y of the band's 64x64 slab in a chunk (16-byte pieces, the wave's 4 KB in
+// one go); 64 ballots turn the rows into the slab's 64 column masks (bit y of
+// column x: voxel (x, y) solid), which are wave-uniform, so the placement
+// recurrence runs in scalar registers.  A placed voxel raises its diagonal's
+// entry of an LDS copy of the row with atomicMax: a band places at most once
+// per diagonal (the diagonal's bit stays in T until it drops out below y = 0)
+// and later bands carry larger heights, so the maximum is the value the
+// band-after-band overwrite leaves.
+constexpr int kShadowBS = 512;
+__global__ void __launch_bounds__(kShadowBS) k_shadow_rows64(const uint8_t *voxels, uint8_t mb0,
+                                                            uint8_t mb1, uint8_t mb2,
+                                                            uint16_t *map, uint32_t sx)
+{
+    constexpr int R = 64;
+    extern __shared__ uint32_t row32[];    // sx entries
+    const uint32_t mx = 1u << mb0, my = 1u << mb1, Z = (1u << mb2) * R;
+    const uint32_t lane = threadIdx.x & 63u, wv = threadIdx.x >> 6;
+    constexpr uint32_t NW = kShadowBS / 64;
+    for (uint32_t wz = blockIdx.x; wz < Z; wz += gridDim.x) {
+        for (uint32_t i = threadIdx.x; i < sx; i += kShadowBS) row32[i] = 0u;
+        __syncthreads();
+        const uint32_t cz = wz >> 6, zl = wz & 63u;
+        for (uint32_t cy = wv; cy < my; cy += NW) {
+            uint64_t T = 0;
+            const uint32_t wy = cy * R + lane;
+            for (uint32_t cx = 0; cx < mx; cx++) {
+                const uint32_t chunk = ((cz << mb1 | cy) << mb0) | cx;
+                const uint4 *src = reinterpret_cast<const uint4 *>(
+                    voxels + ((size_t)chunk << 18) + ((size_t)zl * R + lane) * R);
+                const uint4 v[4] = {src[0], src[1], src[2], src[3]};
+                // bit x of this lane's row: voxel (x, lane) != 0
+                uint64_t rowbits = 0;
+#pragma unroll
+                for (int q = 0; q < 4; q++) {
+                    const uint32_t d[4] = {v[q].x, v[q].y, v[q].z, v[q].w};
+#pragma unroll
+                    for (int k = 0; k < 4; k++) {
+                        // high bit of each byte: set iff the byte is not 0
+                        const uint32_t nz = (((d[k] & 0x7F7F7F7Fu) + 0x7F7F7F7Fu) | d[k]) & 0x80808080u;
+                        // bytes 0..3 -> bits 0..3 of this dword's nibble
+                        const uint32_t n4 = ((nz >> 7) & 1u) | ((nz >> 14) & 2u) | ((nz >> 21) & 4u) |
+                                            ((nz >> 28) & 8u);
+                        rowbits |= (uint64_t)n4 << (16 * q + 4 * k);
+                    }
+                }
+                const uint32_t base = cx * R + wy;     // diagonal of (x = 0, y = lane)
+#pragma unroll 8
+                for (int x = 0; x < R; x++) {
+                    const uint64_t col = __ballot((rowbits >> x) & 1ull);
+                    const uint64_t placed = col & ~T & ~(T >> 1);
+                    T = (T | placed) >> 1;
+                    if ((placed >> lane) & 1ull) atomicMax(&row32[base + (uint32_t)x], wy);
+                }
+            }
+        }
+        __syncthreads();
+        uint16_t *dst = map + (size_t)sx * wz;
+        for (uint32_t i = threadIdx.x; i < sx; i += kShadowBS) dst[i] = (uint16_t)row32[i];
+        __syncthreads();
+    }
+}
+
 struct PairList {
     uint32_t chunk;
     uint32_t first, count;   // range in the sphere-index array
@@ -402,6 +466,15 @@ extern "C" int vp_gpu_shadow_build(vp_ctx *ctx, const vp_world_desc *world,
     const uint64_t len = vp_shadow_padded_len((uint32_t)sx, (uint32_t)Z);
     hipLaunchKernelGGL(k_fill16, dim3(2048), dim3(256), 0, st, d_map, len);
     VP_HIP_TRY(hipGetLastError());
+    if (b == 6 && sx * 4 <= 120 * 1024) {
+        // 64^3 chunks: every band's chain at once (a u32 row in LDS)
+        const int grid = (int)std::min<uint64_t>(Z, (uint64_t)ctx->num_cus * 2u);
+        hipLaunchKernelGGL(k_shadow_rows64, dim3(grid), dim3(kShadowBS), sx * sizeof(uint32_t), st,
+                           d_voxels, world->max_bitw[0], world->max_bitw[1], world->max_bitw[2],
+                           d_map, (uint32_t)sx);
+        VP_HIP_TRY(hipGetLastError());
+        return VP_OK;
+    }
     const int grid = (int)std::min<uint64_t>(Z, (uint64_t)ctx->num_cus * 4u);
     const size_t lds = sx * sizeof(uint16_t);
     VP_HIP_TRY(by_root((int)b, [&](auto bc) {
