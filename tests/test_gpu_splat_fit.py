@@ -158,3 +158,29 @@ def test_render_with_block_bounds_matches_plain(gpu, scene):
         ia.backward(G)
         ib.backward(G)
         assert torch.allclose(a.grad, b.grad, rtol=1e-5, atol=1e-6)
+
+
+def test_render_views_matches_single_views(gpu, scene):
+    """All views in one Function (the fit's path): each image equals the
+    single-view render bit for bit, and the one gradient buffer equals the
+    sum of the single views' gradients (float atomics, any order)."""
+    import torch
+    from voxplat_amd import splat_fit as F
+
+    w, splats, cams = scene
+    n = splats.shape[0]
+    g = torch.Generator(device="cuda").manual_seed(17)
+    rgb = torch.rand((n, 3), generator=g, device="cuda")
+    bounds = gpu.splat_bounds(splats)
+    a = rgb.clone().requires_grad_(True)
+    imgs = F.render_views(gpu, a, splats, cams, bounds=bounds)
+    G = torch.rand_like(imgs)
+    imgs.backward(G)
+    b = rgb.clone().requires_grad_(True)
+    total = 0
+    for v, cam in enumerate(cams):
+        iv = F.render(gpu, b, splats, cam, bounds=bounds)
+        assert torch.equal(iv, imgs[v].detach())
+        total = total + (iv * G[v]).sum()
+    total.backward()
+    assert torch.allclose(a.grad, b.grad, rtol=1e-5, atol=1e-6)

@@ -52,6 +52,47 @@ class SplatColour(torch.autograd.Function):
         return grad.to(ctx.dtype), None, None, None, None, None, None
 
 
+class SplatColourViews(torch.autograd.Function):
+    """images[V, H, W, 3] = splat(rgb[n, 3]) from V cameras of one size.
+    The backward scatters every view's dL/dimage into one [n, 3] gradient,
+    so a step over V views holds one dense gradient instead of V."""
+
+    @staticmethod
+    def forward(ctx, rgb, gpu: GPU, splats, cams, lod=None, lod_log2: int = 0, bounds=None):
+        if rgb.dim() != 2 or rgb.shape[1] != 3 or rgb.shape[0] != splats.shape[0]:
+            raise ValueError(f"rgb must be [{splats.shape[0]}, 3], got {tuple(rgb.shape)}")
+        W_, H_ = cams[0].width, cams[0].height
+        if any(c.width != W_ or c.height != H_ for c in cams):
+            raise ValueError("all views must have the same size")
+        dev = rgb.device
+        V = len(cams)
+        rgb_c = rgb.detach().to(torch.float32).contiguous()
+        depth = torch.empty((H_, W_), dtype=torch.float32, device=dev)
+        ids = torch.empty((V, H_, W_), dtype=torch.int32, device=dev)
+        img = torch.empty((V, H_, W_, 3), dtype=torch.float32, device=dev)
+        wt = torch.empty((V, H_, W_), dtype=torch.float32, device=dev)
+        for v, cam in enumerate(cams):
+            gpu.splat(splats, cam, lod=lod, lod_log2=lod_log2, voxel_rgb=rgb_c, rgb=True,
+                      weight=True, bounds=bounds, out=(depth, ids[v], img[v], wt[v]))
+        ctx.save_for_backward(ids, wt)
+        ctx.gpu, ctx.cams, ctx.n, ctx.dtype = gpu, cams, rgb.shape[0], rgb.dtype
+        return img.to(rgb.dtype)
+
+    @staticmethod
+    def backward(ctx, g_img):
+        ids, wt = ctx.saved_tensors
+        g = g_img.to(torch.float32).contiguous()
+        grad = torch.zeros((ctx.n, 3), dtype=torch.float32, device=g_img.device)
+        for v, cam in enumerate(ctx.cams):
+            ctx.gpu.splat_backward(ctx.n, cam, ids[v], wt[v], g[v], grad)
+        return grad.to(ctx.dtype), None, None, None, None, None, None
+
+
+def render_views(gpu: GPU, rgb, splats, cams, lod=None, lod_log2: int = 0, bounds=None):
+    """All views at once ([V, H, W, 3]); one gradient buffer in the backward."""
+    return SplatColourViews.apply(rgb, gpu, splats, tuple(cams), lod, lod_log2, bounds)
+
+
 def render(gpu: GPU, rgb, splats, cam, lod=None, lod_log2: int = 0, bounds=None):
     """bounds: the list's block bounds (GPU.splat_bounds), computed once while
     the splats stay put; the resolve then culls and orders blocks per view."""
@@ -97,13 +138,14 @@ def fit_colours(gpu: GPU, splats, cams, targets, init, steps: int = 200, lr: flo
     rgb = init.detach().clone().to(torch.float32).requires_grad_(True)
     opt = torch.optim.Adam([rgb], lr=lr, betas=(0.9, 0.999))
     hist = torch.zeros(steps, dtype=torch.float64, device=init.device)
+    tgt = torch.stack(list(targets))
     for it in range(steps):
         opt.zero_grad(set_to_none=True)
-        for cam, tgt in zip(cams, targets):
-            img = render(gpu, rgb, splats, cam, bounds=bounds)
-            l = ((img - tgt) ** 2).mean() / len(cams)
-            l.backward()
-            hist[it] += l.detach()
+        img = render_views(gpu, rgb, splats, cams, bounds=bounds)
+        # the mean over views of each view's mean squared error (equal sizes)
+        l = ((img - tgt) ** 2).mean()
+        l.backward()
+        hist[it] += l.detach()
         opt.step()
         with torch.no_grad():
             rgb.clamp_(0.0, 1.0)
