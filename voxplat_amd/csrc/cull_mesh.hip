@@ -108,7 +108,18 @@ struct Lay : Rows<B, V2> {
     static constexpr int SWIN = 2 * P::R + 2;
     static constexpr int WAVE_WORDS = (4 * LCAP + 2 * SWIN + 7) / 8;
     static constexpr int WAVES = SOFF + SOFF_WORDS;
-    static constexpr int TOTAL = WAVES + NW * WAVE_WORDS;
+    // LOD colours: n(l)^3 bytes per level 1-4, cell (x, y, z) at (z n + y) n + x.
+    // Built once per chunk right after the chunk was read (its lines are still
+    // in L2), so the emission of levels 1-4 reads colours from LDS.  Only the
+    // 1024-thread R = 64 kernel without v2 flags has the LDS room (41.7 KiB).
+    static constexpr bool MIP = (B == 6 && BS == 1024 && !V2);
+    static constexpr int cube(int v) { return v * v * v; }
+    static constexpr int cbase(int l)
+    {
+        return l <= 1 ? 0 : cbase(l - 1) + cube(Geo<B>::n(l - 1));
+    }
+    static constexpr int COL = WAVES + NW * WAVE_WORDS;
+    static constexpr int TOTAL = COL + (MIP ? (cbase(5) + 7) / 8 : 0);
 };
 
 __device__ __forceinline__ uint64_t compress_pairs(uint64_t r)
@@ -539,6 +550,94 @@ __device__ __forceinline__ void l1_child_planes(const uint64_t *lds, int s, int 
           (~pdz & ~pdy & hi[0][0]);
 }
 
+// Child planes of a level-l row (l >= 2) from its four child rows c[dz][dy]
+// of level l - 1 (zero past the child grid): for every cell X the greatest
+// (z, y, x) non-zero child, as bit planes dz, dy, dx over X.
+__device__ __forceinline__ void lod_child_planes(const uint64_t ch[2][2], uint64_t &pdz,
+                                                 uint64_t &pdy, uint64_t &pdx)
+{
+    uint64_t hi[2][2], an[2][2];
+#pragma unroll
+    for (int dz = 0; dz < 2; dz++)
+#pragma unroll
+        for (int dy = 0; dy < 2; dy++) {
+            hi[dz][dy] = even_bits(ch[dz][dy] >> 1);
+            an[dz][dy] = even_bits(ch[dz][dy]) | hi[dz][dy];
+        }
+    pdz = an[1][1] | an[1][0];
+    pdy = (pdz & an[1][1]) | (~pdz & an[0][1]);
+    pdx = (pdz & pdy & hi[1][1]) | (pdz & ~pdy & hi[1][0]) | (~pdz & pdy & hi[0][1]) |
+          (~pdz & ~pdy & hi[0][0]);
+}
+
+// Level-1 colours of the level-1 row (Y, Z), Y, Z <= R/2, R = 64: for every
+// cell X the colour of its greatest (z, y, x) visible level-0 child, the value
+// the downsample's last write leaves there (mesher.c:460-493).  The four child
+// rows (y = 2Y + dy, z = 2Z + dz) are read in 16-byte loads, from the chunk
+// or, at y = R or z = R, from the first plane of the +y / +z neighbour; the
+// cell X = R/2 takes its voxel from the +x neighbour.  m: the row's bits.
+template <int B>
+__device__ __forceinline__ void col1_row(const ChunkRefs &c, const uint64_t *lds, uint8_t *dst,
+                                         int Y, int Z, uint64_t m)
+{
+    constexpr int R = 1 << B, H = R / 2;
+    static_assert(R == 64, "16-byte row pieces of a 64-voxel row");
+    uint64_t pdz, pdy, pdx;
+    l1_child_planes<B, false>(lds, Z, Y, pdz, pdy, pdx);
+    const uint8_t *src[4];      // child row q = 2 dz + dy, or its uniform value
+    uint32_t fill[4];
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        const int y = 2 * Y + (q & 1), z = 2 * Z + (q >> 1);
+        const uint8_t *p = nullptr;
+        int u = 0;
+        if (y < R && z < R) {
+            p = c.own + (size_t)(z * R + y) * R;
+            u = c.own_u;
+        } else if (y == R && z < R) {
+            u = c.nb_u[1];
+            if (u < 0) p = c.nb[1] + (size_t)z * R * R;
+        } else if (z == R && y < R) {
+            u = c.nb_u[2];
+            if (u < 0) p = c.nb[2] + (size_t)y * R;
+        }
+        src[q] = u < 0 ? p : nullptr;
+        fill[q] = (uint32_t)(u < 0 ? 0 : u) * 0x01010101u;
+    }
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+        if (!((m >> (16 * h)) & 0xFFFFull)) continue;
+        uint32_t w[4][8];
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            if (src[q]) {
+                const uint4 v0 = *reinterpret_cast<const uint4 *>(src[q] + 32 * h);
+                const uint4 v1 = *reinterpret_cast<const uint4 *>(src[q] + 32 * h + 16);
+                w[q][0] = v0.x; w[q][1] = v0.y; w[q][2] = v0.z; w[q][3] = v0.w;
+                w[q][4] = v1.x; w[q][5] = v1.y; w[q][6] = v1.z; w[q][7] = v1.w;
+            } else {
+#pragma unroll
+                for (int k = 0; k < 8; k++) w[q][k] = fill[q];
+            }
+        }
+        // cell X = 16 h + i: children x = 2X + dx are bytes 2i + dx of the half
+#pragma unroll
+        for (int i = 0; i < 16; i++) {
+            const int X = 16 * h + i;
+            const bool bz = (pdz >> X) & 1ull, by = (pdy >> X) & 1ull;
+            const uint32_t bx = (uint32_t)(pdx >> X) & 1u;
+            const uint32_t t1 = by ? w[3][i >> 1] : w[2][i >> 1];
+            const uint32_t t0 = by ? w[1][i >> 1] : w[0][i >> 1];
+            const uint32_t wd = bz ? t1 : t0;
+            dst[X] = (uint8_t)(wd >> (16 * (i & 1) + 8 * bx));
+        }
+    }
+    if ((m >> H) & 1ull) {
+        const int dz = (int)((pdz >> H) & 1ull), dy = (int)((pdy >> H) & 1ull);
+        dst[H] = (uint8_t)cell_colour<B>(c, R, 2 * Y + dy, 2 * Z + dz);
+    }
+}
+
 // Emission of one chunk whose slice offsets are in LDS.  Records are ordered
 // level, z, y, x, so each z-slice of each level is a contiguous range of the
 // output starting at its slice offset.  Waves take slices from the work
@@ -608,7 +707,7 @@ __device__ void emit_slices(const CullArgs &a, const ChunkRefs &c, uint64_t *lds
                 uint32_t pos = off - b0;
                 uint64_t m = rb;
                 const uint32_t code = ln << 7;
-                if (l == 1) {
+                if (!L::MIP && l == 1) {
                     // level 1: the descent to the level-0 child for the whole row
                     // at once, as bit planes over X (the child's dz, dy, dx)
                     l1_child_planes<B, V2>(lds, s, (int)ln, pdz, pdy, pdx);
@@ -663,6 +762,7 @@ __device__ void emit_slices(const CullArgs &a, const ChunkRefs &c, uint64_t *lds
                     }
                 }
             } else {
+                const uint8_t *colm = reinterpret_cast<const uint8_t *>(lds + L::COL) + L::cbase(l);
 #pragma unroll 1
                 for (uint32_t j0 = 0; j0 < nb; j0 += 64 * U) {
                     uint32_t col[U];
@@ -676,14 +776,18 @@ __device__ void emit_slices(const CullArgs &a, const ChunkRefs &c, uint64_t *lds
                             const uint32_t e = list[j];
                             const int X = (int)(e & 127u), Y = (int)((e >> 7) & 127u);
                             int qx = X, qy = Y, qz = s;
-                            if (l == 1) {
-                                qz = 2 * s + (int)((e >> 16) & 1u);
-                                qy = 2 * Y + (int)((e >> 17) & 1u);
-                                qx = 2 * X + (int)((e >> 18) & 1u);
+                            if constexpr (L::MIP) {
+                                col[u] = colm[(s * nl + Y) * nl + X];
                             } else {
-                                descend_bf<B, V2>(lds, l, qx, qy, qz);
+                                if (l == 1) {
+                                    qz = 2 * s + (int)((e >> 16) & 1u);
+                                    qy = 2 * Y + (int)((e >> 17) & 1u);
+                                    qx = 2 * X + (int)((e >> 18) & 1u);
+                                } else {
+                                    descend_bf<B, V2>(lds, l, qx, qy, qz);
+                                }
+                                col[u] = cell_colour<B>(c, qx, qy, qz);
                             }
-                            col[u] = cell_colour<B>(c, qx, qy, qz);
                             const uint32_t wy = c.org[1] + ((uint32_t)Y << l);
                             const uint32_t ex = (uint32_t)(X + Y) << l;
                             uint32_t sh = 0;
@@ -858,7 +962,12 @@ __device__ void cull_chunk(const CullArgs &a, uint32_t k, uint32_t n, uint64_t *
                             if (Z < R && Y < R) bxa |= (uint32_t)(lds[L::XB + Z] >> Y) & 1u;
                         }
                     }
-                    lds[L::P1 + r] = compress_pairs(acc) | ((uint64_t)bxa << (R / 2));
+                    const uint64_t row = compress_pairs(acc) | ((uint64_t)bxa << (R / 2));
+                    lds[L::P1 + r] = row;
+                    if constexpr (L::MIP)
+                        if (row)
+                            col1_row<B>(c, lds, reinterpret_cast<uint8_t *>(lds + L::COL) + r * n1,
+                                        py, pz, row);
                 }
             }
             __syncthreads();
@@ -869,17 +978,36 @@ __device__ void cull_chunk(const CullArgs &a, uint32_t k, uint32_t n, uint64_t *
                 const int src = L::P1 + (G::row_base(l - 1) - G::RB1);
                 for (int r = threadIdx.x; r < nc * nc; r += BS) {
                     const int pz = r / nc, py = r % nc;
-                    uint64_t acc = 0;
-                    for (int dz = 0; dz < 2; dz++) {
-                        const int Z = 2 * pz + dz;
-                        if (Z >= np) continue;
+                    uint64_t ch[2][2];
+#pragma unroll
+                    for (int dz = 0; dz < 2; dz++)
+#pragma unroll
                         for (int dy = 0; dy < 2; dy++) {
-                            const int Y = 2 * py + dy;
-                            if (Y >= np) continue;
-                            acc |= lds[src + Z * np + Y];
+                            const int Z = 2 * pz + dz, Y = 2 * py + dy;
+                            ch[dz][dy] = (Z < np && Y < np) ? lds[src + Z * np + Y] : 0ull;
+                        }
+                    const uint64_t row = compress_pairs(ch[0][0] | ch[0][1] | ch[1][0] | ch[1][1]);
+                    lds[dst + r] = row;
+                    if constexpr (L::MIP) {
+                        // colour of cell X: the level-(l-1) colour of its greatest
+                        // non-zero child, which is where descending from X ends
+                        if (row) {
+                            uint64_t pdz, pdy, pdx;
+                            lod_child_planes(ch, pdz, pdy, pdx);
+                            const uint8_t *cs = reinterpret_cast<const uint8_t *>(lds + L::COL) +
+                                                L::cbase(l - 1);
+                            uint8_t *cd = reinterpret_cast<uint8_t *>(lds + L::COL) + L::cbase(l) +
+                                          r * nc;
+                            uint64_t mm = row;
+                            while (mm) {
+                                const int X = __builtin_ctzll(mm);
+                                mm &= mm - 1ull;
+                                const int dz = (int)((pdz >> X) & 1ull), dy = (int)((pdy >> X) & 1ull),
+                                          dx = (int)((pdx >> X) & 1ull);
+                                cd[X] = cs[((2 * pz + dz) * np + 2 * py + dy) * np + 2 * X + dx];
+                            }
                         }
                     }
-                    lds[dst + r] = compress_pairs(acc);
                 }
                 __syncthreads();
             }
