@@ -4,8 +4,9 @@
 Headline (one step): a full remesh -- cull, level-0 splat list and LOD levels
 1..4 -- of every chunk of the 16384x512x16384 uint8 world (524,288 chunks of
 64^3, 128 GiB resident in HBM, seeded value-noise terrain with 5% caves), as
-one batched persistent kernel captured in a hipGraph, the list ordered
-heaviest first.  Inputs are resident in HBM before timing starts.
+one batched persistent kernel captured in a hipGraph, the list in locality
+order (mixed chunks in chunk-index order, the lightest of them last, then the
+uniform chunks).  Inputs are resident in HBM before timing starts.
 
 Also measured (extra keys, not the headline value):
   * edit_frame: on the same world, frames of 10k sphere edits each followed by
@@ -110,7 +111,7 @@ def leg_world16k(args) -> dict:
         "data": "synthetic (seeded value-noise terrain, 5% caves)",
         "config": {"workload": "remesh L0-L4 of 16384x512x16384 (524,288 chunks of 64^3, "
                                "128 GiB resident), batched persistent kernel in a hipGraph, "
-                               "list heaviest first",
+                               "list in locality order",
                    "chunks": full["chunks"], "occupancy": res["occupancy"],
                    "splat_records": full["records"], "launch": full["launch"]},
         "roofline": {"bound": "hbm", "achieved": full["gb_s"], "peak": HBM_PEAK_GBS,
@@ -164,11 +165,11 @@ def leg_world2k(args) -> dict:
     sx, sz = w.shadow_dims()
     smap = torch.empty(abi.shadow_padded_len(sx, sz), dtype=torch.int16, device=dev)
     gpu.shadow_build(w, vox, smap)
-    # the list is ordered heaviest chunk first (the output follows the list;
-    # every chunk's records are found through its offset)
-    from voxplat_amd.world_frame import heaviest_first
+    # the list is in locality order (the output follows the list; every
+    # chunk's records are found through its offset)
+    from voxplat_amd.world_frame import locality_order
 
-    ids, offs, counts = heaviest_first(gpu, w, vox, torch.arange(nch, dtype=torch.int32, device=dev),
+    ids, offs, counts = locality_order(gpu, w, vox, torch.arange(nch, dtype=torch.int32, device=dev),
                                        smap, uni)
     torch.cuda.synchronize()
     total_items = int(offs[-1].item())

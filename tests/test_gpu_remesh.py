@@ -79,9 +79,42 @@ def test_launch_shapes(gpu, block, wg, u):
         gpu.set_remesh_shape(None, 512, 2, 4)
 
 
+def test_locality_order(gpu):
+    """locality_order: a permutation of the list with the mixed chunks first
+    (index order, then the `tail` lightest, heaviest of them first), the
+    uniform chunks last; the remesh of the reordered list is the oracle's
+    for that list."""
+    import torch
+
+    from voxplat_amd.world_frame import locality_order
+
+    cfg = W.small_world(6, (2, 2, 2), 0x5EED007A, hmax=70)
+    w = cfg.desc()
+    vox_d = torch.empty(w.chunks * w.R ** 3, dtype=torch.uint8, device="cuda")
+    uni_d = torch.empty(w.chunks, dtype=torch.int16, device="cuda")
+    gpu.generate(w, cfg.gen(), vox_d, uni_d)
+    sx, sz = w.shadow_dims()
+    smap_d = torch.empty(abi.shadow_padded_len(sx, sz), dtype=torch.int16, device="cuda")
+    gpu.shadow_build(w, vox_d, smap_d)
+    ids0 = torch.arange(w.chunks - 1, -1, -1, dtype=torch.int32, device="cuda")
+    ids, offs, counts = locality_order(gpu, w, vox_d, ids0, smap_d, uni_d, tail=2)
+    torch.cuda.synchronize()
+    ids_h, uni_h = ids.cpu().numpy(), uni_d.cpu().numpy()
+    assert sorted(ids_h.tolist()) == list(range(w.chunks))
+    mixed = uni_h[ids_h] < 0
+    nm = int(mixed.sum())
+    assert 3 < nm < w.chunks, "the test world needs mixed and uniform chunks"
+    assert mixed[:nm].all() and not mixed[nm:].any()
+    assert np.all(np.diff(ids_h[:nm - 2]) > 0) and np.all(np.diff(ids_h[nm:]) > 0)
+    tot = counts.cpu().numpy().view(np.uint32).sum(1)
+    assert tot[nm - 2] >= tot[nm - 1] and min(tot[nm - 2:nm]) <= min(tot[:nm - 2])
+    vox, smap = vox_d.cpu().numpy(), smap_d.cpu().numpy().view(np.uint16)
+    _check(w, vox, ids_h.astype(np.uint32), smap, gpu_remesh(gpu, w, vox, ids_h.astype(np.uint32), smap))
+
+
 def test_residency_and_autotuner(gpu):
     """Resident workgroups per CU follow the kernels' LDS (about 67 KiB at
-    512 threads, 151 KiB at 1024); the autotuner times only resident shapes,
+    512 threads, 124 KiB at 1024); the autotuner times only resident shapes,
     keeps the fastest as the world's own shape, and leaves the default and
     other worlds alone; its pick remeshes to the oracle's bytes."""
     import torch

@@ -3,7 +3,7 @@
     python -m voxplat_amd.remesh_run [--world 2k|16k] [--launches K] [--copy-gib G]
                                      [--kernel cull|mesh|nodes]
 
-Generates the world on the GPU, orders the chunk list heaviest first and
+Generates the world on the GPU, puts the chunk list in locality order and
 sizes the output with counting passes, then issues K direct (not
 graph-captured) launches of vp_gpu_cull_mesh over every chunk, so that a
 kernel tracer or counter collector sees one dispatch per launch.
@@ -26,7 +26,7 @@ import torch
 from . import abi
 from . import workload as W
 from ._lib import GPU
-from .world_frame import heaviest_first, mesh_bytes
+from .world_frame import locality_order, mesh_bytes
 
 
 def main(argv=None) -> int:
@@ -57,7 +57,7 @@ def main(argv=None) -> int:
     ids = torch.arange(w.chunks, dtype=torch.int32, device="cuda")
     if a.kernel == "mesh":
         return run_mesh(gpu, cfg, w, vox, uni, smap, ids, a.launches)
-    ids, offs, counts = heaviest_first(gpu, w, vox, ids, smap, uni)
+    ids, offs, counts = locality_order(gpu, w, vox, ids, smap, uni)
     torch.cuda.synchronize()
     if a.kernel == "nodes":
         return run_nodes(gpu, cfg, w, vox, uni, smap, ids, offs, counts, a.launches)

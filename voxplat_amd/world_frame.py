@@ -98,6 +98,36 @@ def heaviest_first(gpu: GPU, w: abi.WorldDesc, voxels, ids, smap, uniform, lod: 
     return ids, offs, counts
 
 
+def locality_order(gpu: GPU, w: abi.WorldDesc, voxels, ids, smap, uniform, lod: bool = True,
+                   tail: int | None = None):
+    """Order a chunk list for the batched remesh, and size it: the mixed chunks
+    in chunk-index order, the lightest `tail` of them (default two per CU)
+    moved to the end, heaviest of those first, then the uniform chunks.
+
+    The batched remesh hands chunks to workgroups in list order, so the order
+    is the caller's schedule.  In chunk-index order a mixed chunk's +x
+    neighbour is the next chunk, read by another workgroup at about the same
+    time: the 4096 lines the +x column touches are then mostly found in the
+    caches instead of memory.  All mixed chunks first keeps the memory system
+    busy without gaps, and the light chunks at the end leave no long chunk
+    running alone when the list runs dry.  Returns (ids, offsets, counts) for
+    the new order, offsets and counts from a counting pass over it."""
+    ids = torch.sort(ids).values
+    _, offs, counts = gpu.cull_mesh(w, voxels, ids, shadow_map=smap, uniform=uniform, lod=lod,
+                                    count_only=True)
+    mixed = uniform[ids.long()].to(torch.int32) < 0 if uniform is not None else \
+        torch.ones_like(ids, dtype=torch.bool)
+    mids, weight = ids[mixed], counts.sum(1)[mixed]
+    k = min(2 * gpu.num_cus if tail is None else tail, int(mids.numel()))
+    light = torch.argsort(weight, descending=True, stable=True)[mids.numel() - k:]
+    keep = torch.ones(mids.numel(), dtype=torch.bool, device=ids.device)
+    keep[light] = False
+    ids = torch.cat([mids[keep], mids[light], ids[~mixed]]).contiguous()
+    _, offs, counts = gpu.cull_mesh(w, voxels, ids, shadow_map=smap, uniform=uniform, lod=lod,
+                                    count_only=True)
+    return ids, offs, counts
+
+
 class Frame:
     """The per-frame pass of a resident world, captured once (FrameGraph) and
     replayed each frame: every launch rebuilds the dirty and idle lists from
@@ -208,13 +238,13 @@ def rle_bytes(n_chunks: int, words: int, R3: int = 262144) -> int:
 def full_remesh(world: ResidentWorld, steps: int = 3, warmup: int = 1, log=None,
                 autotune: bool = False):
     """Remesh every chunk of the world (L0-L4) as one graph launch per step,
-    the list ordered heaviest first; with `autotune`, the launch shape is
+    the list in locality order (`locality_order`); with `autotune`, the launch shape is
     first tuned on this list and kept as the world's shape (the frames' graph
     then uses it too).  Returns timings; the record buffer is freed before
     returning."""
     gpu, w = world.gpu, world.w
     ids = torch.arange(w.chunks, dtype=torch.int32, device=world.voxels.device)
-    ids, offs, counts = heaviest_first(gpu, w, world.voxels, ids, world.smap, world.uniform)
+    ids, offs, counts = locality_order(gpu, w, world.voxels, ids, world.smap, world.uniform)
     torch.cuda.synchronize()
     records = int(offs[-1].item()) // 4
     if log:
