@@ -23,6 +23,8 @@
 //  5. emission: every z-slice of every level is a contiguous range of the
 //     output; waves take slices from a work counter in LDS and write them
 //     without workgroup barriers (emit_slices).
+#include <type_traits>
+
 #include "vp_internal.h"
 #include "../../include/vp_host.h"
 
@@ -102,12 +104,6 @@ struct Lay : Rows<B, V2> {
     static constexpr int SOFF = P::PEND;
     static constexpr int ITEM_U32 = P::NSLICES + 1;
     static constexpr int SOFF_WORDS = (P::NSLICES + 2 + 1) / 2;
-    // per wave: LCAP list entries (u32: cell code, colour) and SWIN shadow entries
-    static constexpr int mini(int a, int b) { return a < b ? a : b; }
-    static constexpr int LCAP = mini(512, P::N0 * P::N0);
-    static constexpr int SWIN = 2 * P::R + 2;
-    static constexpr int WAVE_WORDS = (4 * LCAP + 2 * SWIN + 7) / 8;
-    static constexpr int WAVES = SOFF + SOFF_WORDS;
     // LOD colours: n(l)^3 bytes per level 1-4, cell (x, y, z) at (z n + y) n + x.
     // Built once per chunk right after the chunk was read (its lines are still
     // in L2), so the emission of levels 1-4 reads colours from LDS.  Only the
@@ -118,6 +114,16 @@ struct Lay : Rows<B, V2> {
     {
         return l <= 1 ? 0 : cbase(l - 1) + cube(Geo<B>::n(l - 1));
     }
+    // per wave: LCAP list entries and SWIN shadow entries.  An entry is the
+    // cell code y << 7 | x, u16 in the MIP kernel (which keeps no level-1
+    // child selection in it, so its lists hold 2048 cells: most slices in
+    // one batch), else u32 with the selection in bits 16-18.
+    static constexpr int mini(int a, int b) { return a < b ? a : b; }
+    static constexpr int EB = MIP ? 2 : 4;
+    static constexpr int LCAP = mini(MIP ? 2048 : 512, P::N0 * P::N0);
+    static constexpr int SWIN = 2 * P::R + 2;
+    static constexpr int WAVE_WORDS = (EB * LCAP + 2 * SWIN + 7) / 8;
+    static constexpr int WAVES = SOFF + SOFF_WORDS;
     static constexpr int COL = WAVES + NW * WAVE_WORDS;
     static constexpr int TOTAL = COL + (MIP ? (cbase(5) + 7) / 8 : 0);
 };
@@ -658,7 +664,8 @@ __device__ void emit_slices(const CullArgs &a, const ChunkRefs &c, uint64_t *lds
     constexpr int LCAP = L::LCAP, SWIN = L::SWIN;
     const uint32_t ln = lane_id(), w = threadIdx.x >> 6;
     uint32_t *soff = reinterpret_cast<uint32_t *>(lds + L::SOFF);
-    uint32_t *list = reinterpret_cast<uint32_t *>(lds + L::WAVES + (int)w * L::WAVE_WORDS);
+    using LE = typename std::conditional<L::MIP, uint16_t, uint32_t>::type;
+    LE *list = reinterpret_cast<LE *>(lds + L::WAVES + (int)w * L::WAVE_WORDS);
     uint16_t *win = reinterpret_cast<uint16_t *>(list + LCAP);
     const int nitems = a.lod ? L::NSLICES : L::SLB1;
 #pragma unroll 1
@@ -717,19 +724,19 @@ __device__ void emit_slices(const CullArgs &a, const ChunkRefs &c, uint64_t *lds
                         const uint32_t sel = (uint32_t)((pdz >> x) & 1ull) |
                                              ((uint32_t)((pdy >> x) & 1ull) << 1) |
                                              ((uint32_t)((pdx >> x) & 1ull) << 2);
-                        list[pos++] = code | x | (sel << 16);
+                        list[pos++] = (LE)(code | x | (sel << 16));
                     }
                 } else {
                     while (m) {
                         const uint32_t x = (uint32_t)__builtin_ctzll(m);
                         m &= m - 1ull;
-                        list[pos++] = code | x;
+                        list[pos++] = (LE)(code | x);
                     }
-                    if (xb) list[pos] = code | (uint32_t)R;
+                    if (xb) list[pos] = (LE)(code | (uint32_t)R);
                 }
             }
             if (rbx && b1 == T && tot64 >= b0 && ((rbx >> ln) & 1ull))
-                list[tot64 - b0 + rank_below(rbx)] = (64u << 7) | ln;
+                list[tot64 - b0 + rank_below(rbx)] = (LE)((64u << 7) | ln);
             wave_sync();
             const uint32_t nb = b1 - b0;
             uint64_t *ob = o + b0;
