@@ -668,28 +668,59 @@ __device__ void emit_slices(const CullArgs &a, const ChunkRefs &c, uint64_t *lds
     LE *list = reinterpret_cast<LE *>(lds + L::WAVES + (int)w * L::WAVE_WORDS);
     uint16_t *win = reinterpret_cast<uint16_t *>(list + LCAP);
     const int nitems = a.lod ? L::NSLICES : L::SLB1;
+    auto take = [&]() -> uint32_t {
+        uint32_t t = 0;
+        if (ln == 0)
+            t = __hip_atomic_fetch_add(&soff[L::ITEM_U32], 1u, __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_WORKGROUP);
+        return uni((uint32_t)__shfl((int)t, 0, 64));
+    };
+    auto level_of = [](uint32_t t) {
+        return (int)t < L::SLB1 ? 0 : (int)t < L::SLB2 ? 1 : (int)t < L::SLB3 ? 2
+             : (int)t < L::SLB4 ? 3 : 4;
+    };
+    auto slice_of = [](uint32_t t, int lv) {
+        return (int)t - (lv == 0 ? L::SLB0 : lv == 1 ? L::SLB1 : lv == 2 ? L::SLB2
+                        : lv == 3 ? L::SLB3 : L::SLB4);
+    };
+    // The shadow-map row a slice samples (2R + 2 entries, lane e holds e,
+    // e + 64, e + 128) is loaded into registers one item ahead, so its wait
+    // overlaps the previous slice's work.
+    constexpr int WQ = (SWIN + 63) / 64;
+    auto fetch_win = [&](uint32_t t, uint32_t (&wv)[WQ]) {
+#pragma unroll
+        for (int q = 0; q < WQ; q++) wv[q] = 0;
+        if (!a.map || (int)t >= nitems || soff[t + 1] == soff[t]) return;
+        const int lv = level_of(t), sl = slice_of(t, lv);
+        const uint32_t dd = lv ? (1u << lv) : 0u;
+        const size_t wb = (size_t)a.sx * (c.org[2] + ((uint32_t)sl << lv) + dd) +
+                          c.org[0] + c.org[1] + 2u * dd;
+#pragma unroll
+        for (int q = 0; q < WQ; q++)
+            if ((int)ln + 64 * q < SWIN) wv[q] = a.map[wb + ln + 64 * q];
+    };
+    uint32_t nxt = take();
+    uint32_t wv[WQ];
+    fetch_win(nxt, wv);
 #pragma unroll 1
     for (;;) {
-        uint32_t it = 0;
-        if (ln == 0)
-            it = __hip_atomic_fetch_add(&soff[L::ITEM_U32], 1u, __ATOMIC_RELAXED,
-                                        __HIP_MEMORY_SCOPE_WORKGROUP);
-        it = uni((uint32_t)__shfl((int)it, 0, 64));
+        const uint32_t it = nxt;
         if ((int)it >= nitems) break;
-        const int l = (int)it < L::SLB1 ? 0 : (int)it < L::SLB2 ? 1 : (int)it < L::SLB3 ? 2
-                    : (int)it < L::SLB4 ? 3 : 4;
-        const int s = (int)it - (l == 0 ? L::SLB0 : l == 1 ? L::SLB1 : l == 2 ? L::SLB2
-                                 : l == 3 ? L::SLB3 : L::SLB4);
+        const int l = level_of(it), s = slice_of(it, l);
         const int nl = G::n(l);
         const uint32_t sbeg = soff[it], T = soff[it + 1] - sbeg;
-        if (T == 0) continue;
         const uint32_t d = l ? (1u << l) : 0u;        // LOD samples at +(1 << l) (mesher.c:523-531)
-        wave_sync();                                   // the previous slice's window reads are done
-        if (a.map) {
-            const size_t wb = (size_t)a.sx * (c.org[2] + ((uint32_t)s << l) + d) +
-                              c.org[0] + c.org[1] + 2u * d;
-            for (int e = (int)ln; e < SWIN; e += 64) win[e] = a.map[wb + e];
+        if (T != 0) {
+            wave_sync();                               // the previous slice's window reads are done
+            if (a.map) {
+#pragma unroll
+                for (int q = 0; q < WQ; q++)
+                    if ((int)ln + 64 * q < SWIN) win[ln + 64 * q] = (uint16_t)wv[q];
+            }
         }
+        nxt = take();
+        fetch_win(nxt, wv);
+        if (T == 0) continue;
         const uint64_t *rows = l == 0 ? lds + L::V0 + s * N0
                                       : lds + L::P1 + (G::row_base(l) - G::RB1) + s * nl;
         uint64_t rb = 0;
