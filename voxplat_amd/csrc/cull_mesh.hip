@@ -896,6 +896,9 @@ __device__ void cull_chunk(const CullArgs &a, uint32_t k, uint32_t n, uint64_t *
     uint32_t total = 0, lvl[5] = {0, 0, 0, 0, 0};
     if (!trivial) {
         // ---- 1. occupancy rows and neighbour planes ----
+        // (and the slice counters, which phases 2-3 add into)
+        for (int i = threadIdx.x; i < L::NSLICES; i += BS)
+            reinterpret_cast<uint32_t *>(lds + L::SOFF)[i] = 0u;
         if (c.own_u >= 0) {
             const uint64_t v = c.own_u ? RM : 0ull;
             for (int r = threadIdx.x; r < R * R; r += BS) lds[L::S + r] = v;
@@ -932,7 +935,9 @@ __device__ void cull_chunk(const CullArgs &a, uint32_t k, uint32_t n, uint64_t *
 
         // ---- 2. level-0 visibility (mesher.c:421-448 as row algebra) ----
         // Each thread computes its rows into registers; V0 overwrites S only
-        // after the barrier, when every S row has been read.
+        // after the barrier, when every S row has been read.  The rows'
+        // cells are added to their slice's counter.
+        uint32_t *soff = reinterpret_cast<uint32_t *>(lds + L::SOFF);
         {
             constexpr int RPT = (R * R + BS - 1) / BS;
             uint64_t vrow[RPT];
@@ -959,6 +964,14 @@ __device__ void cull_chunk(const CullArgs &a, uint32_t k, uint32_t n, uint64_t *
                 const uint64_t bal = __ballot(bx);
                 const int ln = (int)lane_id();
                 if (act && (ln % R) == 0) lds[L::XB + (r >> B)] = (bal >> ln) & RM;
+                uint32_t cv = act ? (uint32_t)__popcll(v) + bx : 0u;
+                if constexpr (R == 64) {   // a wave's rows are one slice's
+#pragma unroll
+                    for (int d = 32; d >= 1; d >>= 1) cv += __shfl_xor(cv, d, 64);
+                    if (act && ln == 0 && cv) atomicAdd(&soff[r >> B], cv);
+                } else {
+                    if (cv) atomicAdd(&soff[r >> B], cv);
+                }
             }
             // cells (x, R, z): +y neighbour voxel exposed to this chunk's (x, R-1, z);
             // cells (x, y, R): +z neighbour voxel exposed to (x, y, R-1)
@@ -967,6 +980,8 @@ __device__ void cull_chunk(const CullArgs &a, uint32_t k, uint32_t n, uint64_t *
             if (t < R) {
                 ey = lds[L::HY + t] & ~lds[L::S + t * R + (R - 1)] & RM;
                 ez = lds[L::HZ + t] & ~lds[L::S + (R - 1) * R + t] & RM;
+                if (ey) atomicAdd(&soff[t], (uint32_t)__popcll(ey));
+                if (ez) atomicAdd(&soff[R], (uint32_t)__popcll(ez));
             }
             __syncthreads();
 #pragma unroll
@@ -1002,6 +1017,7 @@ __device__ void cull_chunk(const CullArgs &a, uint32_t k, uint32_t n, uint64_t *
                     }
                     const uint64_t row = compress_pairs(acc) | ((uint64_t)bxa << (R / 2));
                     lds[L::P1 + r] = row;
+                    if (row) atomicAdd(&soff[L::SLB1 + pz], (uint32_t)__popcll(row));
                     if constexpr (L::MIP)
                         if (row)
                             col1_row<B>(c, lds, reinterpret_cast<uint8_t *>(lds + L::COL) + r * n1,
@@ -1026,6 +1042,9 @@ __device__ void cull_chunk(const CullArgs &a, uint32_t k, uint32_t n, uint64_t *
                         }
                     const uint64_t row = compress_pairs(ch[0][0] | ch[0][1] | ch[1][0] | ch[1][1]);
                     lds[dst + r] = row;
+                    if (row)
+                        atomicAdd(&soff[(l == 2 ? L::SLB2 : l == 3 ? L::SLB3 : L::SLB4) + pz],
+                                  (uint32_t)__popcll(row));
                     if constexpr (L::MIP) {
                         // colour of cell X: the level-(l-1) colour of its greatest
                         // non-zero child, which is where descending from X ends
@@ -1054,33 +1073,12 @@ __device__ void cull_chunk(const CullArgs &a, uint32_t k, uint32_t n, uint64_t *
             __syncthreads();
         }
 
-        // ---- 4. per-slice counts -> scan -> slice offsets ----
+        // ---- 4. slice counts (added up in phases 2-3) -> scan -> slice offsets ----
         // Records are ordered by level, z, y, x, so every z-slice of every
         // level owns a contiguous range: one offset per slice is enough.
-        uint32_t *soff = reinterpret_cast<uint32_t *>(lds + L::SOFF);
         {
             const int w = (int)(threadIdx.x >> 6);
             const uint32_t ln = lane_id();
-            for (int sl = w; sl < L::NSLICES; sl += NW) {
-                const int l = sl < L::SLB1 ? 0 : sl < L::SLB2 ? 1 : sl < L::SLB3 ? 2
-                            : sl < L::SLB4 ? 3 : 4;
-                const int nl = G::n(l);
-                const int z = sl - (l == 0 ? L::SLB0 : l == 1 ? L::SLB1 : l == 2 ? L::SLB2
-                                    : l == 3 ? L::SLB3 : L::SLB4);
-                uint32_t cv = 0;
-                for (int y = (int)ln; y < nl; y += 64) {
-                    if (l == 0) {
-                        cv += __popcll(lds[L::V0 + z * N0 + y]);
-                        if (z < R && y < R) cv += (uint32_t)(lds[L::XB + z] >> y) & 1u;
-                    } else {
-                        cv += __popcll(lds[L::P1 + (G::row_base(l) - G::RB1) + z * nl + y]);
-                    }
-                }
-#pragma unroll
-                for (int d = 32; d >= 1; d >>= 1) cv += __shfl_xor(cv, d, 64);
-                if (ln == 0) soff[sl] = cv;
-            }
-            __syncthreads();
             if (w == 0) {   // exclusive scan of the slice counts in one wave
                 constexpr int KS = (L::NSLICES + 63) / 64;
                 uint32_t part = 0;
