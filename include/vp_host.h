@@ -89,6 +89,10 @@ int vp_host_remesh_graph_destroy(vp_remesh_graph *graph);
  *      the look-back that places each stream also sizes it, so one pass
  *      writes d_word_offsets (entries + 1) and the words that fit
  *      capacity_words (VP_STATUS_OVERFLOW otherwise), and d_idle_uniform.
+ *      A chunk that d_uniform marks uniform (the value every voxel holds;
+ *      the table the remesh takes, which edits mark mixed) gets its one-run
+ *      stream without a read of its voxels, as the reference keeps all-air
+ *      chunks on the shared null-chunk stream.
  * Arrays indexed by list entry must hold `chunks` entries (chunks + 1 for
  * offsets, 5 * chunks for level counts).  The graph owns its scratch (list
  * counts, tickets and look-back words of both persistent kernels) and frees

@@ -208,7 +208,7 @@ extern "C" int vp_host_frame_graph_create(vp_ctx *ctx, const vp_world_desc *worl
             if (crc == VP_OK)
                 crc = vp::rle_encode_launch(ctx, world, d_voxels, out->d_idle_ids, nch,
                                             out->d_counts + 1, out->d_words, out->capacity_words,
-                                            out->d_word_offsets, out->d_idle_uniform,
+                                            out->d_word_offsets, out->d_idle_uniform, d_uniform,
                                             g->d_ticket + 16, g->d_state + nch, s);
         }
         hipError_t e = hipStreamEndCapture(s, &g->graph);

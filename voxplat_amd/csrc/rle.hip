@@ -36,6 +36,7 @@ struct RleArgs {
     uint64_t        cap;
     uint64_t       *offsets;
     int16_t        *uniform_out;
+    const int16_t  *uniform_in;    // nullable: a chunk marked uniform here is not read (encode)
     uint32_t       *ticket;
     uint64_t       *state;
     uint32_t       *status;
@@ -118,6 +119,13 @@ __device__ void rle_encode_chunk(const RleArgs &a, uint32_t k, uint32_t n, uint6
         chunk = 0xFFFFFFFFu;
     }
     const uint8_t *src = chunk == 0xFFFFFFFFu ? nullptr : a.voxels + ((size_t)chunk << (3 * B));
+    // A chunk the caller's uniform table marks uniform (every voxel holds that
+    // value; edits mark their chunks mixed) is one run and the terminator:
+    // none of its voxels is read.
+    const int uval = (src && a.uniform_in) ? (int)a.uniform_in[chunk] : -1;
+    const bool fast = uval >= 0;
+    uint32_t runs = 1;
+    if (!fast) {
 
     // ---- 1. start bits ----
     // DEPTH 16-byte loads per thread in flight; the byte before a piece is
@@ -168,7 +176,6 @@ __device__ void rle_encode_chunk(const RleArgs &a, uint32_t k, uint32_t n, uint6
         mine += (uint32_t)__popcll(b);
         if (b && first == 0xFFFFFFFFu) first = (uint32_t)(w * 64 + __builtin_ctzll(b));
     }
-    uint32_t runs = 0;
     uint32_t run = r_scan_excl<BS>(mine, wsum, runs);
     uint32_t nxt = r_suffix_min_excl<BS>(first, wsum);
     if (nxt == 0xFFFFFFFFu) nxt = VOX;
@@ -187,6 +194,7 @@ __device__ void rle_encode_chunk(const RleArgs &a, uint32_t k, uint32_t n, uint6
         run += (uint32_t)__popcll(bm[w]);
     }
     __syncthreads();
+    }   // !fast
 
     // ---- 3. publish the word count (runs + terminator), look back ----
     const uint64_t nwords = (uint64_t)runs + 1u;
@@ -225,7 +233,8 @@ __device__ void rle_encode_chunk(const RleArgs &a, uint32_t k, uint32_t n, uint6
             a.offsets[k] = excl;
             if (k + 1 == n) a.offsets[n] = excl + nwords;
             if (a.uniform_out)
-                a.uniform_out[k] = (runs == 1) ? (int16_t)(src ? src[0] : 0) : (int16_t)-1;
+                a.uniform_out[k] = fast ? (int16_t)uval
+                                 : (runs == 1) ? (int16_t)(src ? src[0] : 0) : (int16_t)-1;
             if (a.words && excl + nwords > a.cap) atomicOr(a.status, (uint32_t)VP_STATUS_OVERFLOW);
         }
     }
@@ -240,7 +249,8 @@ __device__ void rle_encode_chunk(const RleArgs &a, uint32_t k, uint32_t n, uint6
     // step 1, so each is a memory round trip), then their G words stored.
     constexpr int G = 8;
     uint32_t *out = a.words + base;
-    for (int w = threadIdx.x; w < WORDS; w += BS) {
+    if (fast && threadIdx.x == 0) out[0] = ((uint32_t)uval << 24) | (uint32_t)VOX;
+    for (int w = threadIdx.x; w < (fast ? 0 : WORDS); w += BS) {
         uint64_t b = bm[w];
         uint32_t o = woff[w];
         const uint32_t wn = wnext[w];
@@ -464,7 +474,8 @@ static hipError_t rle_dispatch(uint32_t root_bitw, int dir, const RleArgs &a, in
 int rle_encode_launch(vp_ctx *ctx, const vp_world_desc *world, const uint8_t *d_voxels,
                       const uint32_t *d_chunk_ids, uint32_t n, const uint32_t *d_n,
                       uint32_t *d_words, uint64_t capacity_words, uint64_t *d_word_offsets,
-                      int16_t *d_uniform_out, uint32_t *ticket, uint64_t *state, hipStream_t st)
+                      int16_t *d_uniform_out, const int16_t *d_uniform_in, uint32_t *ticket,
+                      uint64_t *state, hipStream_t st)
 {
     int rc = rle_check(world);
     if (rc) return rc;
@@ -480,6 +491,7 @@ int rle_encode_launch(vp_ctx *ctx, const vp_world_desc *world, const uint8_t *d_
     a.cap = capacity_words;
     a.offsets = d_word_offsets;
     a.uniform_out = d_uniform_out;
+    a.uniform_in = d_uniform_in;
     a.status = ctx->d_status;
     a.single = 0;
     a.nchunks = (uint32_t)(1ull << (world->max_bitw[0] + world->max_bitw[1] + world->max_bitw[2]));

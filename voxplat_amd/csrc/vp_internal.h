@@ -104,11 +104,14 @@ int cull_mesh_launch(vp_ctx *ctx, const vp_world_desc *world, const uint8_t *d_v
                      uint64_t *state, hipStream_t st);
 
 // The RLE encode of vp_gpu_rle with explicit scratch (a ticket word and n
-// look-back words) and an optional device-resident count d_n <= n.
+// look-back words), an optional device-resident count d_n <= n and an
+// optional uniform table (d_uniform_in, per chunk: the value every voxel
+// holds, or -1): a chunk marked uniform there is encoded without reading it.
 int rle_encode_launch(vp_ctx *ctx, const vp_world_desc *world, const uint8_t *d_voxels,
                       const uint32_t *d_chunk_ids, uint32_t n, const uint32_t *d_n,
                       uint32_t *d_words, uint64_t capacity_words, uint64_t *d_word_offsets,
-                      int16_t *d_uniform_out, uint32_t *ticket, uint64_t *state, hipStream_t st);
+                      int16_t *d_uniform_out, const int16_t *d_uniform_in, uint32_t *ticket,
+                      uint64_t *state, hipStream_t st);
 
 // Stores of output the kernel never reads back: streamed past the caches
 // (non-temporal), so they do not evict the inputs still being read.

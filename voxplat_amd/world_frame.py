@@ -229,10 +229,13 @@ def mesh_bytes(uniform_h: np.ndarray, face_codes: np.ndarray, R: int = 64) -> in
     return mixed * R ** 3 + n * ((R + 2) ** 3 - R ** 3) + 56 * faces + maps + 8 * n
 
 
-def rle_bytes(n_chunks: int, words: int, R3: int = 262144) -> int:
-    """Algorithmic bytes of one RLE encode: every voxel read once, every word
-    (terminators included) written once, one u64 offset and i16 value per chunk."""
-    return n_chunks * (R3 + 10) + 4 * words
+def rle_bytes(n_chunks: int, words: int, R3: int = 262144, read_chunks: int | None = None) -> int:
+    """Algorithmic bytes of one RLE encode (or decode): the voxels of every
+    chunk read (written) once -- of `read_chunks` of them when the others are
+    uniform chunks the encoder does not read --, every word (terminators
+    included) written (read) once, one u64 offset and i16 value per chunk."""
+    nread = n_chunks if read_chunks is None else read_chunks
+    return nread * R3 + n_chunks * 10 + 4 * words
 
 
 def full_remesh(world: ResidentWorld, steps: int = 3, warmup: int = 1, log=None,
@@ -367,8 +370,11 @@ def run_world16k(frames: int = 4, edits: int = 10000, log=None, keep: bool = Fal
         nd, ni = int(fr.dirty.numel()), int(fr.idle.numel())
         dirty_uni = world.uniform[fr.dirty.long()].cpu().numpy()
         rb = remesh_bytes(dirty_uni, fr.counts.cpu().numpy().view(np.uint32))
-        lb = rle_bytes(ni, fr.words_n, world.R3)
+        # idle chunks the uniform table marks uniform are encoded without a read
+        idle_read = int((world.uniform[fr.idle.long()] < 0).sum().item()) if ni else 0
+        lb = rle_bytes(ni, fr.words_n, world.R3, read_chunks=idle_read)
         per.append({"edit_s": round(t_edit, 3), "dirty_chunks": nd, "idle_chunks": ni,
+                    "idle_chunks_read": idle_read,
                     "records": fr.records, "rle_words": fr.words_n,
                     "ms": round(max(ms, wall), 3), "graph_ms": round(ms, 3),
                     "algorithmic_bytes": rb + lb,
