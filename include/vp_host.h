@@ -81,7 +81,8 @@ int vp_host_remesh_graph_destroy(vp_remesh_graph *graph);
  * chunkset.c:213-230,300-306).  Each replay
  *   1. turns the dirty bytes d_dirty[chunks] (set by vp_gpu_edit_spheres) into
  *      d_dirty_ids (the dirty chunks, ascending) and d_idle_ids (every other
- *      chunk, ascending), with their counts in d_counts[0] and d_counts[1],
+ *      chunk: those d_uniform marks mixed, ascending, then those it marks
+ *      uniform, ascending), with their counts in d_counts[0] and d_counts[1],
  *      and clears the dirty bytes;
  *   2. remeshes the dirty list as vp_gpu_cull_mesh does with d_n = d_counts:
  *      records to remesh.d_splats, offsets and level counts per entry;

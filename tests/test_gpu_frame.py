@@ -37,12 +37,16 @@ def test_frame_graph_bursts(gpu):
             if spheres:
                 world.edit(spheres)
             exp = host(world.dirty).astype(bool)
+            marked_mixed = host(world.uniform) < 0
             fr.run()
             fr.sync()
             assert gpu.status() == 0
             assert not host(world.dirty).any()                       # flags cleared
             assert np.array_equal(host(fr.dirty), np.flatnonzero(exp).astype(np.int32))
-            assert np.array_equal(host(fr.idle), np.flatnonzero(~exp).astype(np.int32))
+            # idle: the chunks the uniform table marks mixed, then the uniform ones
+            want_idle = np.concatenate([np.flatnonzero(~exp & marked_mixed),
+                                        np.flatnonzero(~exp & ~marked_mixed)]).astype(np.int32)
+            assert np.array_equal(host(fr.idle), want_idle)
             nd = int(fr.dirty.numel())
             assert (nd > 0) == (f != 3)
             vh, mh = host(world.voxels), host(world.smap, np.uint16)

@@ -5,9 +5,11 @@
 // chunkset_manage (src/chunkset.c:246-313) walks every chunk each frame: a
 // dirty chunk has its dirty flag cleared and is remeshed, any other chunk is
 // compressed (chunk_compress, chunkset.c:213-230,300-306).  Here the walk is
-// two small kernels that turn the per-chunk dirty bytes into a dirty list and
-// an idle list on the device (both ascending, as the loop visits them) and
-// clear the flags; the batched remesh then reads the dirty count and the RLE
+// two small kernels that turn the per-chunk dirty bytes into a dirty list
+// (ascending, as the loop visits them) and an idle list (the chunks the
+// uniform table marks mixed, ascending, then the uniform ones, ascending: the
+// encoder reads the first and not the second, and so streams its reads
+// without gaps) on the device and clear the flags; the batched remesh then reads the dirty count and the RLE
 // encode the idle count from device memory, so one captured graph serves
 // every frame, whatever the edits marked.  The graph owns its scratch: the
 // list kernels' per-workgroup counts, and a ticket word and look-back words
@@ -23,8 +25,18 @@ constexpr int kListBS = 256;
 constexpr int kListPer = 4;                       // chunks per thread
 constexpr int kListChunks = kListBS * kListPer;   // chunks per workgroup
 
-// Per workgroup: (dirty, clean) chunks of its range, packed as d | c << 16.
-__global__ void __launch_bounds__(kListBS) k_frame_count(const uint8_t *dirty, uint32_t nch,
+// Kind of chunk i: 1 dirty, 2 idle and marked mixed (or no table), 3 idle
+// and marked uniform.
+__device__ __forceinline__ uint32_t list_kind(const uint8_t *dirty, const int16_t *uniform,
+                                              uint32_t i)
+{
+    return dirty[i] ? 1u : (!uniform || uniform[i] < 0) ? 2u : 3u;
+}
+
+// Per workgroup: (dirty, idle mixed) chunks of its range, packed as
+// d | m << 16; its idle uniform chunks are the rest of its range.
+__global__ void __launch_bounds__(kListBS) k_frame_count(const uint8_t *dirty,
+                                                          const int16_t *uniform, uint32_t nch,
                                                           uint32_t *blk)
 {
     __shared__ uint32_t wsum[kListBS / 64];
@@ -32,7 +44,10 @@ __global__ void __launch_bounds__(kListBS) k_frame_count(const uint8_t *dirty, u
     uint32_t v = 0;
     for (int j = 0; j < kListPer; j++) {
         const uint32_t i = base + threadIdx.x * kListPer + j;
-        if (i < nch) v += dirty[i] ? 1u : (1u << 16);
+        if (i < nch) {
+            const uint32_t k = list_kind(dirty, uniform, i);
+            v += k == 1u ? 1u : k == 2u ? (1u << 16) : 0u;
+        }
     }
 #pragma unroll
     for (int d = 32; d >= 1; d >>= 1) v += (uint32_t)__shfl_xor((int)v, d, 64);
@@ -45,13 +60,14 @@ __global__ void __launch_bounds__(kListBS) k_frame_count(const uint8_t *dirty, u
     }
 }
 
-// Ordered scatter: chunk i goes to its place among the dirty or the clean
-// chunks (ascending), and its dirty byte is cleared.  counts[0] = dirty,
-// counts[1] = clean.  The workgroups read their counts before any byte is
-// cleared: k_frame_count ran first, as its own launch.
-__global__ void __launch_bounds__(kListBS) k_frame_scatter(uint8_t *dirty, uint32_t nch,
-                                                            const uint32_t *blk, uint32_t nblk,
-                                                            uint32_t *dirty_ids,
+// Ordered scatter: chunk i goes to its place among the dirty chunks, the idle
+// mixed ones or (after all those) the idle uniform ones, each ascending, and
+// its dirty byte is cleared.  counts[0] = dirty, counts[1] = idle.  The
+// workgroups read their counts before any byte is cleared: k_frame_count ran
+// first, as its own launch.
+__global__ void __launch_bounds__(kListBS) k_frame_scatter(uint8_t *dirty, const int16_t *uniform,
+                                                            uint32_t nch, const uint32_t *blk,
+                                                            uint32_t nblk, uint32_t *dirty_ids,
                                                             uint32_t *idle_ids, uint32_t *counts)
 {
     __shared__ uint32_t wsum[4][kListBS / 64];
@@ -103,7 +119,7 @@ __global__ void __launch_bounds__(kListBS) k_frame_scatter(uint8_t *dirty, uint3
     uint32_t mine = 0;
     for (int j = 0; j < kListPer; j++) {
         const uint32_t i = i0 + j;
-        f[j] = i < nch ? (dirty[i] ? 1 : 2) : 0;
+        f[j] = i < nch ? (uint8_t)list_kind(dirty, uniform, i) : 0;
         mine += f[j] == 1 ? 1u : f[j] == 2 ? (1u << 16) : 0u;
     }
     uint32_t x = mine;
@@ -118,19 +134,26 @@ __global__ void __launch_bounds__(kListBS) k_frame_scatter(uint8_t *dirty, uint3
     uint32_t wb = 0;
     for (uint32_t k = 0; k < w; k++) wb += wsum[0][k];
     const uint32_t local = wb + x - mine;          // packed, at most kListChunks per kind
-    uint32_t od = s_pre[0] + (local & 0xFFFFu), oc = s_pre[1] + (local >> 16);
+    const uint32_t ld = local & 0xFFFFu, lm = local >> 16;
+    // idle uniform chunks before this thread's first: those of the earlier
+    // workgroups and threads, which are neither dirty nor idle mixed
+    const uint32_t before = i0 < nch ? i0 : nch;
+    uint32_t od = s_pre[0] + ld, om = s_pre[1] + lm;
+    uint32_t ou = s_tot[1] + (before - s_pre[0] - ld - s_pre[1] - lm);
     for (int j = 0; j < kListPer; j++) {
         const uint32_t i = i0 + j;
         if (f[j] == 1) {
             dirty_ids[od++] = i;
             dirty[i] = 0;
         } else if (f[j] == 2) {
-            idle_ids[oc++] = i;
+            idle_ids[om++] = i;
+        } else if (f[j] == 3) {
+            idle_ids[ou++] = i;
         }
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         counts[0] = s_tot[0];
-        counts[1] = s_tot[1];
+        counts[1] = nch - s_tot[0];
     }
 }
 
@@ -193,10 +216,10 @@ extern "C" int vp_host_frame_graph_create(vp_ctx *ctx, const vp_world_desc *worl
     rc = VP_ERR_HIP;
     if (hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) == hipSuccess) {
         int crc = VP_OK;
-        hipLaunchKernelGGL(vp::k_frame_count, dim3(nblk), dim3(vp::kListBS), 0, s, d_dirty, nch,
+        hipLaunchKernelGGL(vp::k_frame_count, dim3(nblk), dim3(vp::kListBS), 0, s, d_dirty, d_uniform, nch,
                            g->d_blk);
         if (hipGetLastError() != hipSuccess) crc = VP_ERR_HIP;
-        hipLaunchKernelGGL(vp::k_frame_scatter, dim3(nblk), dim3(vp::kListBS), 0, s, d_dirty, nch,
+        hipLaunchKernelGGL(vp::k_frame_scatter, dim3(nblk), dim3(vp::kListBS), 0, s, d_dirty, d_uniform, nch,
                            g->d_blk, nblk, out->d_dirty_ids, out->d_idle_ids, out->d_counts);
         if (hipGetLastError() != hipSuccess) crc = VP_ERR_HIP;
         if (crc == VP_OK) {
