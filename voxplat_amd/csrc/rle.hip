@@ -314,7 +314,10 @@ __global__ void __launch_bounds__(BS) k_rle_decode(RleArgs a)
 {
     constexpr int R = 1 << B, VOX = R * R * R, WORDS = VOX / 64, WPT = (WORDS + BS - 1) / BS;
     constexpr int K = 8;             // stream words per thread per batch, loads in flight
-    constexpr int VCAP = 16384;      // run values kept in LDS for streams of up to VCAP runs
+    // run values kept in LDS for streams of up to VCAP runs: at R = 64 what
+    // leaves two workgroups per CU (2 x 81.8 KB); a chunk with 5% of its
+    // voxels as single-voxel holes has about 26,000 runs
+    constexpr int VCAP = VOX < 32640 ? VOX : 32640;
     constexpr uint32_t LMAX = (uint32_t)VOX + 1u;   // longer runs are malformed; clamped
     __shared__ uint64_t bm[WORDS];
     __shared__ uint32_t rank[WORDS];
