@@ -5,8 +5,8 @@ Headline (one step): a full remesh -- cull, level-0 splat list and LOD levels
 1..4 -- of every chunk of the 16384x512x16384 uint8 world (524,288 chunks of
 64^3, 128 GiB resident in HBM, seeded value-noise terrain with 5% caves), as
 one batched persistent kernel captured in a hipGraph, the list in locality
-order (mixed chunks in chunk-index order, the lightest of them last, then the
-uniform chunks).  Inputs are resident in HBM before timing starts.
+order (mixed chunks in descending chunk-index order, the lightest of them
+last, then the uniform chunks).  Inputs are resident in HBM before timing starts.
 
 Also measured (extra keys, not the headline value):
   * edit_frame: on the same world, frames of 10k sphere edits each followed by

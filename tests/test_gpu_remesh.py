@@ -81,9 +81,9 @@ def test_launch_shapes(gpu, block, wg, u):
 
 def test_locality_order(gpu):
     """locality_order: a permutation of the list with the mixed chunks first
-    (index order, then the `tail` lightest, heaviest of them first), the
-    uniform chunks last; the remesh of the reordered list is the oracle's
-    for that list."""
+    (descending index order, then the `tail` lightest, heaviest of them
+    first), the uniform chunks last; the remesh of the reordered list is the
+    oracle's for that list."""
     import torch
 
     from voxplat_amd.world_frame import locality_order
@@ -105,7 +105,7 @@ def test_locality_order(gpu):
     nm = int(mixed.sum())
     assert 3 < nm < w.chunks, "the test world needs mixed and uniform chunks"
     assert mixed[:nm].all() and not mixed[nm:].any()
-    assert np.all(np.diff(ids_h[:nm - 2]) > 0) and np.all(np.diff(ids_h[nm:]) > 0)
+    assert np.all(np.diff(ids_h[:nm - 2]) < 0) and np.all(np.diff(ids_h[nm:]) < 0)
     tot = counts.cpu().numpy().view(np.uint32).sum(1)
     assert tot[nm - 2] >= tot[nm - 1] and min(tot[nm - 2:nm]) <= min(tot[:nm - 2])
     vox, smap = vox_d.cpu().numpy(), smap_d.cpu().numpy().view(np.uint16)
@@ -283,6 +283,14 @@ def test_cfg1_full_world_batched(gpu):
     ids = np.arange(w.chunks, dtype=np.uint32)
     got = gpu_remesh(gpu, w, vox, ids, smap, uniform=uni)
     _check(w, vox_h, ids, smap_h, got)
+    # the list in descending order: every chunk's records and counts equal
+    # those of the list above, at the chunk's place in the new order
+    rd, od, cd, st = gpu_remesh(gpu, w, vox, ids[::-1].copy(), smap, uniform=uni)
+    ra, oa, ca, _ = got
+    assert st == 0 and od[-1] == oa[-1]
+    for e, c in enumerate(ids[::-1]):
+        assert np.array_equal(cd[e], ca[c])
+        assert np.array_equal(rd[od[e] // 4:od[e + 1] // 4], ra[oa[c] // 4:oa[c + 1] // 4]), c
 
 
 def test_output_past_4gib(gpu):

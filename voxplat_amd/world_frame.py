@@ -101,18 +101,19 @@ def heaviest_first(gpu: GPU, w: abi.WorldDesc, voxels, ids, smap, uniform, lod: 
 def locality_order(gpu: GPU, w: abi.WorldDesc, voxels, ids, smap, uniform, lod: bool = True,
                    tail: int | None = None):
     """Order a chunk list for the batched remesh, and size it: the mixed chunks
-    in chunk-index order, the lightest `tail` of them (default two per CU)
-    moved to the end, heaviest of those first, then the uniform chunks.
+    in descending chunk-index order, the lightest `tail` of them (default two
+    per CU) moved to the end, heaviest of those first, then the uniform
+    chunks.
 
     The batched remesh hands chunks to workgroups in list order, so the order
-    is the caller's schedule.  In chunk-index order a mixed chunk's +x
-    neighbour is the next chunk, read by another workgroup at about the same
-    time: the 4096 lines the +x column touches are then mostly found in the
+    is the caller's schedule.  In descending chunk-index order a mixed chunk's
+    +x neighbour is the list's previous entry, read by another workgroup just
+    before: the 4096 lines the +x column touches are then mostly found in the
     caches instead of memory.  All mixed chunks first keeps the memory system
     busy without gaps, and the light chunks at the end leave no long chunk
-    running alone when the list runs dry.  Returns (ids, offsets, counts) for
-    the new order, offsets and counts from a counting pass over it."""
-    ids = torch.sort(ids).values
+    running alone when the list runs dry.  Returns (ids, offsets, counts) for the new
+    order, offsets and counts from a counting pass over it."""
+    ids = torch.sort(ids, descending=True).values
     _, offs, counts = gpu.cull_mesh(w, voxels, ids, shadow_map=smap, uniform=uniform, lod=lod,
                                     count_only=True)
     mixed = uniform[ids.long()].to(torch.int32) < 0 if uniform is not None else \
