@@ -116,16 +116,20 @@ struct Lay : Rows<B, V2> {
     }
     // per wave: LCAP list entries and SWIN shadow entries.  An entry is the
     // cell code y << 7 | x, u16 in the MIP kernel (which keeps no level-1
-    // child selection in it, so its lists hold 2048 cells: most slices in
-    // one batch), else u32 with the selection in bits 16-18.
+    // child selection in it, so its lists hold 1920 cells: most slices in
+    // one batch, and the byte plane below still fits), else u32 with the
+    // selection in bits 16-18.
     static constexpr int mini(int a, int b) { return a < b ? a : b; }
     static constexpr int EB = MIP ? 2 : 4;
-    static constexpr int LCAP = mini(MIP ? 2048 : 512, P::N0 * P::N0);
+    static constexpr int LCAP = mini(MIP ? 1920 : 512, P::N0 * P::N0);
     static constexpr int SWIN = 2 * P::R + 2;
     static constexpr int WAVE_WORDS = (EB * LCAP + 2 * SWIN + 7) / 8;
     static constexpr int WAVES = SOFF + SOFF_WORDS;
     static constexpr int COL = WAVES + NW * WAVE_WORDS;
-    static constexpr int TOTAL = COL + (MIP ? (cbase(5) + 7) / 8 : 0);
+    // the +x neighbour's x = 0 plane as bytes, voxel (0, y, z) at z R + y:
+    // the colours of the cells at x = R, kept from the plane's reads
+    static constexpr int XCB = COL + (MIP ? (cbase(5) + 7) / 8 : 0);
+    static constexpr int TOTAL = XCB + (P::R * P::R + 7) / 8;
 };
 
 __device__ __forceinline__ uint64_t compress_pairs(uint64_t r)
@@ -482,22 +486,28 @@ __device__ __forceinline__ void wave_sync()
 }
 
 // Colour of level-0 cell (x, y, z), x, y, z in [0, R]: one select of the
-// source and at most one byte load.
+// source and at most one byte load, plus one LDS byte read of the +x plane
+// xc, taken when x = R (y, z < R there).  Branch-free: a cell at x = R
+// reads the chunk's first byte instead (a line in the caches), so the
+// record loop stays one path.
 template <int B>
-__device__ __forceinline__ uint32_t cell_colour(const ChunkRefs &c, int x, int y, int z)
+__device__ __forceinline__ uint32_t cell_colour(const ChunkRefs &c, const uint8_t *xc, int x, int y,
+                                                int z)
 {
     constexpr int R = 1 << B;
     const uint8_t *p = c.own;
     int u = c.own_u;
     size_t i = (size_t)((z * R + y) * R + x);
     if (x == R) {
-        p = c.nb[0]; u = c.nb_u[0]; i = (size_t)((z * R + y) * R);
+        i = 0;
     } else if (y == R) {
         p = c.nb[1]; u = c.nb_u[1]; i = (size_t)(z * R * R + x);
     } else if (z == R) {
         p = c.nb[2]; u = c.nb_u[2]; i = (size_t)(y * R + x);
     }
-    return u >= 0 ? (uint32_t)u : (uint32_t)p[i];
+    const uint32_t g = u >= 0 ? (uint32_t)u : (uint32_t)p[i];
+    const uint32_t l = xc[(z * R + y) & (R * R - 1)];
+    return x == R ? l : g;
 }
 
 // Even bits of r (bit 2k -> bit k), 32 of them.
@@ -583,8 +593,8 @@ __device__ __forceinline__ void lod_child_planes(const uint64_t ch[2][2], uint64
 // or, at y = R or z = R, from the first plane of the +y / +z neighbour; the
 // cell X = R/2 takes its voxel from the +x neighbour.  m: the row's bits.
 template <int B>
-__device__ __forceinline__ void col1_row(const ChunkRefs &c, const uint64_t *lds, uint8_t *dst,
-                                         int Y, int Z, uint64_t m)
+__device__ __forceinline__ void col1_row(const ChunkRefs &c, const uint64_t *lds, const uint8_t *xc,
+                                         uint8_t *dst, int Y, int Z, uint64_t m)
 {
     constexpr int R = 1 << B, H = R / 2;
     static_assert(R == 64, "16-byte row pieces of a 64-voxel row");
@@ -640,7 +650,7 @@ __device__ __forceinline__ void col1_row(const ChunkRefs &c, const uint64_t *lds
     }
     if ((m >> H) & 1ull) {
         const int dz = (int)((pdz >> H) & 1ull), dy = (int)((pdy >> H) & 1ull);
-        dst[H] = (uint8_t)cell_colour<B>(c, R, 2 * Y + dy, 2 * Z + dz);
+        dst[H] = xc[(2 * Z + dz) * R + 2 * Y + dy];
     }
 }
 
@@ -665,6 +675,7 @@ __device__ void emit_slices(const CullArgs &a, const ChunkRefs &c, uint64_t *lds
     const uint32_t ln = lane_id(), w = threadIdx.x >> 6;
     uint32_t *soff = reinterpret_cast<uint32_t *>(lds + L::SOFF);
     using LE = typename std::conditional<L::MIP, uint16_t, uint32_t>::type;
+    const uint8_t *xc = reinterpret_cast<const uint8_t *>(lds + L::XCB);
     LE *list = reinterpret_cast<LE *>(lds + L::WAVES + (int)w * L::WAVE_WORDS);
     uint16_t *win = reinterpret_cast<uint16_t *>(list + LCAP);
     const int nitems = a.lod ? L::NSLICES : L::SLB1;
@@ -784,7 +795,7 @@ __device__ void emit_slices(const CullArgs &a, const ChunkRefs &c, uint64_t *lds
                         if (j < nb) {
                             const uint32_t e = list[j];
                             const int x = (int)(e & 127u), y = (int)((e >> 7) & 127u);
-                            col[u] = cell_colour<B>(c, x, y, s);
+                            col[u] = cell_colour<B>(c, xc, x, y, s);
                             const uint32_t wy = c.org[1] + (uint32_t)y;
                             uint32_t sh = 0;
                             if (a.map) sh = !(win[x + y] <= wy && win[x + y + 1] <= wy);
@@ -824,7 +835,7 @@ __device__ void emit_slices(const CullArgs &a, const ChunkRefs &c, uint64_t *lds
                                 } else {
                                     descend_bf<B, V2>(lds, l, qx, qy, qz);
                                 }
-                                col[u] = cell_colour<B>(c, qx, qy, qz);
+                                col[u] = cell_colour<B>(c, xc, qx, qy, qz);
                             }
                             const uint32_t wy = c.org[1] + ((uint32_t)Y << l);
                             const uint32_t ex = (uint32_t)(X + Y) << l;
@@ -917,16 +928,20 @@ __device__ void cull_chunk(const CullArgs &a, uint32_t k, uint32_t n, uint64_t *
         } else {
             load_rows<B, BS>(lds + L::HZ, c.nb[2], R, R);
         }
-        // +x plane: voxel (0, y, z) of the +x neighbour -> HX[z] bit y
+        // +x plane: voxel (0, y, z) of the +x neighbour -> HX[z] bit y, and
+        // the byte itself at XCB[z R + y] (the colour of cell (R, y, z))
+        uint8_t *xcb = reinterpret_cast<uint8_t *>(lds + L::XCB);
         if (c.nb_u[0] >= 0) {
             for (int z = threadIdx.x; z < R; z += BS) lds[L::HX + z] = c.nb_u[0] ? RM : 0ull;
+            for (int i = threadIdx.x; i < R * R; i += BS) xcb[i] = (uint8_t)c.nb_u[0];
         } else {
 #pragma unroll 1
             for (int base = 0; base < R * R; base += BS) {
                 const int i = base + (int)threadIdx.x;
                 const bool act = i < R * R;
-                const uint32_t bit = act ? (c.nb[0][(size_t)i * R] != 0) : 0u;
-                const uint64_t bal = __ballot(bit);
+                const uint32_t v = act ? c.nb[0][(size_t)i * R] : 0u;
+                if (act) xcb[i] = (uint8_t)v;
+                const uint64_t bal = __ballot(v != 0);
                 const int ln = (int)lane_id();
                 if (act && (ln % R) == 0) lds[L::HX + i / R] = (bal >> ln) & RM;
             }
@@ -1020,8 +1035,9 @@ __device__ void cull_chunk(const CullArgs &a, uint32_t k, uint32_t n, uint64_t *
                     if (row) atomicAdd(&soff[L::SLB1 + pz], (uint32_t)__popcll(row));
                     if constexpr (L::MIP)
                         if (row)
-                            col1_row<B>(c, lds, reinterpret_cast<uint8_t *>(lds + L::COL) + r * n1,
-                                        py, pz, row);
+                            col1_row<B>(c, lds, reinterpret_cast<const uint8_t *>(lds + L::XCB),
+                                        reinterpret_cast<uint8_t *>(lds + L::COL) + r * n1, py, pz,
+                                        row);
                 }
             }
             __syncthreads();
