@@ -113,8 +113,8 @@ def test_locality_order(gpu):
 
 
 def test_residency_and_autotuner(gpu):
-    """Resident workgroups per CU follow the kernels' LDS (about 67 KiB at
-    512 threads, 124 KiB at 1024); the autotuner times only resident shapes,
+    """Resident workgroups per CU follow the kernels' LDS (about 70 KiB at
+    512 threads, 156 KiB at 1024); the autotuner times only resident shapes,
     keeps the fastest as the world's own shape, and leaves the default and
     other worlds alone; its pick remeshes to the oracle's bytes."""
     import torch
