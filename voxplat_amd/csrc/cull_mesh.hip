@@ -953,14 +953,22 @@ __device__ void cull_chunk(const CullArgs &a, uint32_t k, uint32_t n, uint64_t *
             for (int z = threadIdx.x; z < R; z += BS) lds[L::HX + z] = c.nb_u[0] ? RM : 0ull;
             for (int i = threadIdx.x; i < R * R; i += BS) xcb[i] = (uint8_t)c.nb_u[0];
         } else {
-#pragma unroll 1
-            for (int base = 0; base < R * R; base += BS) {
-                const int i = base + (int)threadIdx.x;
+            // every byte load first (no branch around them: a lane past the
+            // plane reads its first byte again), then the ballots
+            constexpr int NX = (R * R + BS - 1) / BS;
+            uint32_t v[NX];
+#pragma unroll
+            for (int k = 0; k < NX; k++) {
+                const int i = k * BS + (int)threadIdx.x;
+                v[k] = c.nb[0][(size_t)(i < R * R ? i : 0) * R];
+            }
+            const int ln = (int)lane_id();
+#pragma unroll
+            for (int k = 0; k < NX; k++) {
+                const int i = k * BS + (int)threadIdx.x;
                 const bool act = i < R * R;
-                const uint32_t v = act ? c.nb[0][(size_t)i * R] : 0u;
-                if (act) xcb[i] = (uint8_t)v;
-                const uint64_t bal = __ballot(v != 0);
-                const int ln = (int)lane_id();
+                if (act) xcb[i] = (uint8_t)v[k];
+                const uint64_t bal = __ballot(act && v[k] != 0);
                 if (act && (ln % R) == 0) lds[L::HX + i / R] = (bal >> ln) & RM;
             }
         }
