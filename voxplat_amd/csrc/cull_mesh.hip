@@ -783,15 +783,12 @@ __device__ void emit_slices(const CullArgs &a, const ChunkRefs &c, uint64_t *lds
             const uint32_t nb = b1 - b0;
             uint64_t *ob = o + b0;
             if (l == 0) {
-                // A round's records are stored after the next round's colour
-                // loads are issued: loads and stores share one completion
-                // counter, so a wait for loads also waits for every store
-                // issued before them, and this way a store has a round of
-                // slack instead of none.  Lanes past the batch take its last
+                // The colour loads of a round go out together, with no branch
+                // around them: lanes past the end of the batch take its last
                 // entry and store that entry's record again, to the same
-                // place: no branch around the loads or the stores.
-                uint64_t prv[U];
-                uint32_t pj = 0xFFFFFFFFu;
+                // place.  (With a branch per record the compiler waited for
+                // each load in turn: loads and stores share one completion
+                // counter here, so it cannot wait for a subset.)
                 const uint32_t jmax = nb - 1u;
 #pragma unroll 1
                 for (uint32_t j0 = 0; j0 < nb; j0 += 64 * U) {
@@ -810,22 +807,11 @@ __device__ void emit_slices(const CullArgs &a, const ChunkRefs &c, uint64_t *lds
                         if constexpr (V2) fl = face_flags<B, V2>(lds, x, y, s);
                         rec[u] = pack_record<V2>(c.org[0] + (uint32_t)x, wy, wz, 0u, sh, fl);
                     }
-                    if (pj != 0xFFFFFFFFu) {
-#pragma unroll
-                        for (int u = 0; u < U; u++) {
-                            const uint32_t j = pj + (uint32_t)u * 64u + ln;
-                            __builtin_nontemporal_store(prv[u], &ob[j < jmax ? j : jmax]);
-                        }
-                    }
-#pragma unroll
-                    for (int u = 0; u < U; u++) prv[u] = rec[u] | ((uint64_t)(col[u] & 0xFFu) << 48);
-                    pj = j0;
-                }
-                if (pj != 0xFFFFFFFFu) {
 #pragma unroll
                     for (int u = 0; u < U; u++) {
-                        const uint32_t j = pj + (uint32_t)u * 64u + ln;
-                        __builtin_nontemporal_store(prv[u], &ob[j < jmax ? j : jmax]);
+                        const uint32_t j = j0 + (uint32_t)u * 64u + ln;
+                        __builtin_nontemporal_store(rec[u] | ((uint64_t)(col[u] & 0xFFu) << 48),
+                                                    &ob[j < jmax ? j : jmax]);
                     }
                 }
             } else {
